@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the CFL-less relaxation hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): Maxwell,
+D3Q4 kinetic relaxation scheme, FP64, resonant mode (1,1,1) of the perfectly
+conducting unit cube as exact solution and ghost data, Kuhn mesh 32^3 x 6 =
+196,608 P2 tetrahedra, cfl = 5 (10x the explicit 0.5, lambda = sqrt 3),
+t_end = 2.0 -> 321 steps. A step is one full time step: ghosts at t + dt/2,
+theta = 1/2 transport of the 24 micro-fields (upwind block sweep) and the
+node-local relaxation.
+
+value = DoF-updates/s = (24 * 10 * nelem) * K / (wall time of K steps), with
+the state resident in HBM (no host transfer inside the timed region).
+roofline: the transport sweep (dominant kernel), algorithmic bytes per
+element-step = 480 FP64 values read/written (3840 B) + 112 B geometry + 16 B
+level indices = 3968 B, divided by the sweep's device time (HIP events on
+the solver stream), against 8 TB/s.
+cpu_baseline: the oracle's restatement of the reference CPU algorithm
+(assembled operators, sparse LU per velocity) on a bounded sample of the same
+workload: the first few steps at 32^3, assembly excluded.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dg-cfl-less-maxwell-solver_amd"))
+
+import numpy as np  # noqa: E402
+
+import gdon3d as G  # noqa: E402
+
+METRIC = "DoF-updates/s at fixed L2 error, P2 Maxwell RK3 & relaxation; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
+
+
+def cpu_baseline(n, steps):
+    """Oracle (C99 + OpenMP restatement of the reference) on host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tempfile
+
+    import oracle as O
+
+    path = os.path.join(tempfile.mkdtemp(), f"kuhn{n}.msh")
+    G.load_library().mesh_kuhn_cube_write_msh22(path.encode(), n, 0.0, 0)
+    om = O.OMesh(path)
+    par = np.zeros(16)
+    par[0:3] = np.pi
+    par[3], par[4], par[6] = 1.0, -1.0, np.pi * np.sqrt(3.0)
+    # the reference parallelises the transport over the 4 velocities only
+    threads = min(4, os.cpu_count() or 1)
+    s = O.OSim(om, 1, O.CAVITY_MODE, par, kin=O.D3Q4, threads=threads)
+    s.set_time(2.0, 0.0, 5.0)
+    s.solve(steps)
+    dof = 24 * 10 * om.nelem
+    out = dict(value=dof * steps / s.t_run, unit="DoF-updates/s", cores=threads,
+               kind="port",
+               sample=f"oracle relaxation D3Q4 FP64 on Kuhn {n}^3x6, first {steps} "
+                      f"steps timed ({s.t_run:.2f} s; assembly {s.t_asm:.2f} s excluded); "
+                      f"transport parallel over the 4 velocities, relaxation serial, "
+                      f"as the reference")
+    s.free()
+    om.free()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=316)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the design for sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    lib = G.load_library()
+    if lib.gdh_device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    n = args.n
+    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    mesh = G.Mesh(kuhn=(n, 0.0, 0))
+    model = G.Model("cavity_mode", kinetic="d3q4")
+    sim = G.Simulation(mesh, model)
+    sim.set_time(2.0, 0.0, 5.0)
+    nelem = mesh.nelem
+    dof = 24 * 10 * nelem
+
+    t0 = time.perf_counter()
+    st = G.Stepper(sim)
+    st.sync()
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
+          f"levels={st.levels} setup={t_setup:.2f}s", file=sys.stderr, flush=True)
+
+    st.advance(args.warmup)
+    st.sync()
+    t0 = time.perf_counter()
+    st.advance(args.steps)
+    st.sync()
+    elapsed = time.perf_counter() - t0
+    err = st.error_l2()
+    t_final = st.t
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s, t={t_final:.5f}, "
+          f"L2 error {err:.4e}", file=sys.stderr, flush=True)
+
+    # per-kernel device time (HIP events on the solver stream)
+    st.profile(True)
+    st.advance(args.profile_steps)
+    prof = st.profile(False)
+    sweep_s = prof[0] / args.profile_steps
+    relax_s = prof[1] / args.profile_steps
+    launches = prof[3] / args.profile_steps
+    achieved = SWEEP_BYTES_PER_ELEM * nelem / sweep_s / 1e9
+    print(f"[bench] per step: sweep {sweep_s*1e3:.3f} ms ({launches:.0f} launches), "
+          f"relax {relax_s*1e3:.3f} ms, ghosts {prof[2]/args.profile_steps*1e3:.3f} ms",
+          file=sys.stderr, flush=True)
+    st.free()
+    sim.free()
+
+    value = dof * args.steps / elapsed
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"CFL-less relaxation, D3Q4, FP64, PEC cube mode (1,1,1), "
+                        f"Kuhn {n}^3x6 = {nelem} P2 tets, cfl 5, t_end 2",
+            "elements": nelem,
+            "dof": dof,
+            "itermax": 321 if n == 32 else None,
+            "t_final": t_final,
+            "l2_error": err,
+            "hbm_gb_s_sweep": achieved,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_sweep_level (all levels of one step)",
+            "bytes_per_element_step": SWEEP_BYTES_PER_ELEM,
+            "launches_per_step": launches,
+            "avg_launch_us": sweep_s / max(launches, 1) * 1e6,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

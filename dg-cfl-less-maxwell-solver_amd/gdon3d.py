@@ -1,0 +1,318 @@
+"""ctypes mirror of the gdon3d C API exported by lib/libgdon3d_hip.so.
+
+The structs are field-for-field the reference's gdn_mesh
+(src/mesh/mesh.h:15-45), gdn_model (src/models/model.h:41-62) and
+gdn_simulation (src/simulation/simulation.h:17-38); the functions keep the
+reference names (mesh_read_from_msh22_file, simulation_init, dg_solve, ...).
+This module only marshals arguments: every computation runs in the C/HIP
+library, which aborts with a message when no GPU is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgdon3d_hip.so")
+
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+
+
+class GdnMesh(C.Structure):
+    _fields_ = [
+        ("file_name", C.c_char * 1024),
+        ("nbelems", C.c_int),
+        ("nbnodes", C.c_int),
+        ("nbqnodes_in", C.c_int),
+        ("nbqnodes_out", C.c_int),
+        ("nbfaces", C.c_int),
+        ("nboundaryfaces", C.c_int),
+        ("boundaryface", c_int_p),
+        ("nmacrointerfaces", C.c_int),
+        ("macrointerface", c_int_p),
+        ("is_read", C.c_bool),
+        ("is_built", C.c_bool),
+        ("has_physical_groups", C.c_bool),
+        ("physical_group_is_alloc", C.c_bool),
+        ("phys_group_id_is_alloc", C.c_bool),
+        ("nb_phys_groups", C.c_int),
+        ("phys_group_id", c_int_p),
+        ("physical_group", c_int_p),
+        ("elem2node", c_int_p),
+        ("elem2qnode", c_int_p),
+        ("elem2elem", c_int_p),
+        ("face2elem", c_int_p),
+        ("fnode2enode", c_int_p),
+        ("max_node2elem", C.c_int),
+        ("node2elem", c_int_p),
+        ("node", c_double_p),
+        ("qnode", c_double_p),
+        ("xmin", C.c_double * 3),
+        ("xmax", C.c_double * 3),
+    ]
+
+
+class GdnModel(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("nb_v", C.c_int),
+        ("nb_w", C.c_int),
+        ("vi", c_double_p),
+        ("lambda_", C.c_double),
+        ("get_num_flux", C.c_void_p),
+        ("get_num_flux_boundary", C.c_void_p),
+        ("get_imposed_data", C.c_void_p),
+        ("get_proj_pos", C.c_void_p),
+        ("get_proj_neg", C.c_void_p),
+        ("use_relax_scheme", C.c_bool),
+        ("omega", C.c_double),
+        ("get_w", C.c_void_p),
+        ("get_feq", C.c_void_p),
+        ("get_fR", C.c_void_p),
+    ]
+
+
+class GdnSimulation(C.Structure):
+    _fields_ = [
+        ("m", C.c_int),
+        ("neq", C.c_int),
+        ("wlen", C.c_int),
+        ("itermax", C.c_int),
+        ("use_kinetic_scheme", C.c_bool),
+        ("t", C.c_double),
+        ("tmax", C.c_double),
+        ("dt", C.c_double),
+        ("cfl", C.c_double),
+        ("hmin", C.c_double),
+        ("cpu_running_t", C.c_double),
+        ("cpu_assembly_t", C.c_double),
+        ("wn", c_double_p),
+        ("mdl", C.POINTER(GdnModel)),
+        ("tt", C.POINTER(GdnMesh)),
+    ]
+
+
+# signature of gdn_get_imposed_data, for callbacks written in Python
+IMPOSED_CB = C.CFUNCTYPE(None, c_double_p, C.c_double, c_double_p)
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the library (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (__graft_entry__.build())")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "mesh_read_from_msh22_file": (None, [P(GdnMesh), C.c_char_p]),
+        "mesh_build_connectivity": (None, [P(GdnMesh)]),
+        "mesh_kuhn_cube": (None, [P(GdnMesh), C.c_int, C.c_double, C.c_uint64]),
+        "mesh_kuhn_cube_write_msh22": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_uint64]),
+        "mesh_write_msh22_file": (C.c_int, [P(GdnMesh), C.c_char_p]),
+        "mesh_get_t4_hmin": (C.c_double, [P(GdnMesh)]),
+        "mesh_get_elem_det": (None, [P(GdnMesh), C.c_int, c_double_p]),
+        "mesh_get_elem_codtau": (None, [P(GdnMesh), C.c_int, c_double_p]),
+        "mesh_get_face_vn": (None, [P(GdnMesh), C.c_int, c_double_p]),
+        "gdn_gdn_mesh_free": (None, [P(GdnMesh)]),
+        "simulation_init": (None, [P(GdnMesh), P(GdnModel), P(GdnSimulation)]),
+        "simulation_free": (None, [P(GdnSimulation)]),
+        "simulation_set_time_parameters": (None, [P(GdnSimulation), C.c_double, C.c_double, C.c_double]),
+        "simulation_error_l2": (C.c_double, [P(GdnSimulation)]),
+        "dg_solve": (None, [P(GdnSimulation)]),
+        "dg_solve_rk3": (None, [P(GdnSimulation)]),
+        "dg_solve_relaxation": (None, [P(GdnSimulation)]),
+        "dg_init_sol_macro": (None, [P(GdnSimulation)]),
+        "dg_update_boundary": (None, [P(GdnSimulation), C.c_double]),
+        "dg_apply_relaxation": (None, [P(GdnSimulation)]),
+        "d3q4_set_model": (None, [C.c_void_p]),
+        "d3q6_set_model": (None, [C.c_void_p]),
+        "maxwell_plane_wave_set": (None, [c_double_p, c_double_p, C.c_double]),
+        "maxwell_cavity_mode_set": (None, [C.c_int, C.c_int, C.c_int, c_double_p]),
+        "maxwell_num_flux_upwind": (None, [c_double_p, c_double_p, c_double_p, c_double_p]),
+        "maxwell_num_flux_boundary_metal": (None, [c_double_p, c_double_p, c_double_p, c_double_p]),
+        "gdh_device_count": (C.c_int, []),
+        "gdn_upwind_classes": (C.POINTER(C.c_uint8), [P(GdnMesh), P(GdnModel)]),
+        "dg_stepper_create": (C.c_void_p, [P(GdnSimulation)]),
+        "dg_stepper_init": (None, [C.c_void_p]),
+        "dg_stepper_advance": (None, [C.c_void_p, C.c_int]),
+        "dg_stepper_time": (C.c_double, [C.c_void_p]),
+        "dg_stepper_iterations": (C.c_int, [C.c_void_p]),
+        "dg_stepper_levels": (C.c_int, [C.c_void_p]),
+        "dg_stepper_synchronize": (None, [C.c_void_p]),
+        "dg_stepper_error_l2": (C.c_double, [C.c_void_p]),
+        "dg_stepper_finish": (None, [C.c_void_p]),
+        "dg_stepper_profile": (None, [C.c_void_p, C.c_int, c_double_p]),
+        "dg_stepper_free": (None, [C.c_void_p]),
+        "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
+        "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
+        "t10_get_inv_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
+        "t10_get_internal_ref_entry": (C.c_double, [C.c_int, C.c_int, C.c_int]),
+        "t10_get_face_scale_entry": (C.c_double, [C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def fnptr(name):
+    """Address of an exported C function, for a gdn_model callback field."""
+    lib = load_library()
+    return C.cast(getattr(lib, name), C.c_void_p).value
+
+
+def dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+class Mesh:
+    """A built gdn_mesh owned by Python."""
+
+    def __init__(self, path=None, kuhn=None):
+        lib = load_library()
+        self.s = GdnMesh()
+        if path is not None:
+            lib.mesh_read_from_msh22_file(C.byref(self.s), str(path).encode())
+        else:
+            n, jitter, seed = kuhn
+            lib.mesh_kuhn_cube(C.byref(self.s), n, jitter, seed)
+        lib.mesh_build_connectivity(C.byref(self.s))
+
+    @property
+    def nelem(self):
+        return self.s.nbelems
+
+    def arr(self, field, n, dtype=np.int32):
+        p = getattr(self.s, field)
+        ct = C.c_int if dtype == np.int32 else C.c_double
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).copy()
+
+    def arrays(self):
+        s = self.s
+        nf, nb, ne = s.nbfaces, s.nboundaryfaces, s.nbelems
+        return {
+            "face2elem": self.arr("face2elem", 4 * nf),
+            "fnode2enode": self.arr("fnode2enode", 12 * nf),
+            "elem2qnode": self.arr("elem2qnode", 10 * ne + 6 * nb),
+            "node": self.arr("node", 3 * s.nbqnodes_in, np.float64),
+            "boundaryface": self.arr("boundaryface", nb),
+            "elem2elem": self.arr("elem2elem", 4 * ne),
+        }
+
+    def free(self):
+        if self.s.nbelems:
+            load_library().gdn_gdn_mesh_free(C.byref(self.s))
+
+
+class Model:
+    """A Maxwell gdn_model using the library's callbacks."""
+
+    def __init__(self, imposed="plane_wave", kinetic=None, boundary="upwind",
+                 callback=None, lambda_=1.0):
+        lib = load_library()
+        self.s = GdnModel()
+        self.s.nb_w = 6
+        self.s.lambda_ = lambda_
+        self.s.get_num_flux = fnptr("maxwell_num_flux_upwind")
+        self.s.get_num_flux_boundary = fnptr(
+            "maxwell_num_flux_upwind" if boundary == "upwind" else "maxwell_num_flux_boundary_metal")
+        self._cb = None
+        if callback is not None:
+            self._cb = IMPOSED_CB(callback)
+            self.s.get_imposed_data = C.cast(self._cb, C.c_void_p).value
+        else:
+            self.s.get_imposed_data = fnptr("maxwell_imposed_" + imposed)
+        self.s.get_proj_pos = fnptr("maxwell_get_proj_pos_null_prod_w")
+        self.s.get_proj_neg = fnptr("maxwell_get_proj_neg_prod_w")
+        if kinetic:
+            self.s.use_relax_scheme = True
+            getattr(lib, kinetic + "_set_model")(C.byref(self.s))
+        else:
+            self.s.use_relax_scheme = False
+
+
+def set_plane_wave(k, e0, phase=0.0):
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    e0 = np.ascontiguousarray(e0, dtype=np.float64)
+    load_library().maxwell_plane_wave_set(dptr(k), dptr(e0), phase)
+
+
+def set_cavity_mode(l, m, n, a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    load_library().maxwell_cavity_mode_set(l, m, n, dptr(a))
+
+
+class Simulation:
+    """gdn_simulation on a Mesh and a Model (simulation_init semantics)."""
+
+    def __init__(self, mesh, model):
+        self.mesh, self.model = mesh, model
+        self.s = GdnSimulation()
+        load_library().simulation_init(C.byref(mesh.s), C.byref(model.s), C.byref(self.s))
+
+    def set_time(self, tmax, dt=0.0, cfl=-1.0):
+        load_library().simulation_set_time_parameters(C.byref(self.s), tmax, dt, cfl)
+
+    def solve(self):
+        load_library().dg_solve(C.byref(self.s))
+
+    def wn(self):
+        return np.ctypeslib.as_array(self.s.wn, shape=(self.s.wlen,)).copy()
+
+    def set_wn(self, values):
+        dst = np.ctypeslib.as_array(self.s.wn, shape=(self.s.wlen,))
+        dst[:] = values
+
+    def error_l2(self):
+        return load_library().simulation_error_l2(C.byref(self.s))
+
+    def free(self):
+        """simulation_free also frees the mesh arrays (reference semantics)."""
+        load_library().simulation_free(C.byref(self.s))
+
+
+class Stepper:
+    """dg_stepper_* : set-up, initial condition, k-step advances, timing."""
+
+    def __init__(self, sim):
+        self.lib = load_library()
+        self.sim = sim
+        self.h = self.lib.dg_stepper_create(C.byref(sim.s))
+        self.lib.dg_stepper_init(self.h)
+
+    def advance(self, k):
+        self.lib.dg_stepper_advance(self.h, k)
+
+    def sync(self):
+        self.lib.dg_stepper_synchronize(self.h)
+
+    @property
+    def t(self):
+        return self.lib.dg_stepper_time(self.h)
+
+    @property
+    def levels(self):
+        return self.lib.dg_stepper_levels(self.h)
+
+    def error_l2(self):
+        return self.lib.dg_stepper_error_l2(self.h)
+
+    def profile(self, enable):
+        out = np.zeros(4)
+        self.lib.dg_stepper_profile(self.h, int(enable), dptr(out))
+        return out
+
+    def finish(self):
+        self.lib.dg_stepper_finish(self.h)
+
+    def free(self):
+        if self.h:
+            self.lib.dg_stepper_free(self.h)
+            self.h = None

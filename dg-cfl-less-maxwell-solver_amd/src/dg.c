@@ -1,0 +1,56 @@
+/*
+ * dg_* entry points on the GPU (src/dg/dg.c:23-147). Each one opens a device
+ * context for the simulation, runs the kernel and copies wn back.
+ */
+#include <stdlib.h>
+
+#include <dg/dg.h>
+#include <dg/explicit.h>
+#include <dg/relaxation.h>
+
+#include "host_internal.h"
+
+void dg_solve(gdn_simulation *simu)
+{
+	if (simu->use_kinetic_scheme)
+		dg_solve_relaxation(simu);
+	else
+		dg_solve_rk3(simu);
+}
+
+void dg_init_sol_macro(gdn_simulation *simu)
+{
+	gdh_model_desc md;
+	gdh_ctx *ctx = gdn_ctx_open(simu, &md);
+	double *wex = NULL;
+	if (md.imposed_kind == GDH_IMPOSED_HOST)
+		wex = gdn_eval_imposed_elements(simu, 0.0);
+	gdh_init_solution(ctx, wex);
+	free(wex);
+	gdh_download_state(ctx, simu->wn);
+	gdh_destroy(ctx);
+}
+
+void dg_update_boundary(gdn_simulation *simu, const gdn_real t)
+{
+	gdh_model_desc md;
+	gdh_ctx *ctx = gdn_ctx_open(simu, &md);
+	double *wg = NULL;
+	if (md.imposed_kind == GDH_IMPOSED_HOST)
+		wg = gdn_eval_imposed_ghosts(simu, t);
+	gdh_update_boundary(ctx, t, wg);
+	free(wg);
+	gdh_download_state(ctx, simu->wn);
+	gdh_destroy(ctx);
+}
+
+void dg_apply_relaxation(gdn_simulation *simu)
+{
+	if (!simu->use_kinetic_scheme)
+		gdn_fatal("dg_apply_relaxation needs the kinetic scheme");
+	gdh_model_desc md;
+	gdh_ctx *ctx = gdn_ctx_open(simu, &md);
+	gdh_apply_relaxation(ctx);
+	gdh_download_state(ctx, simu->wn);
+	gdh_destroy(ctx);
+}

@@ -1,0 +1,249 @@
+/*
+ * Host-side preparation of the GPU layer: mesh and model descriptors, the
+ * exact upwind classification, host evaluation of unknown imposed-data
+ * callbacks, and context creation.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mesh/t10.h>
+#include <mesh/tetrageometry.h>
+#include <models/d3q4.h>
+#include <models/d3q6.h>
+#include <models/maxwell.h>
+
+#include "host_internal.h"
+
+void gdn_fatal(const char *msg)
+{
+	fprintf(stderr, "gdon3d: %s\n", msg);
+	abort();
+}
+
+static void *xmalloc(size_t n)
+{
+	void *p = malloc(n ? n : 1);
+	if (!p)
+		gdn_fatal("out of memory");
+	return p;
+}
+
+void gdn_mesh_describe(const gdn_mesh *msh, gdh_mesh_desc *out)
+{
+	if (!msh->is_built)
+		gdn_fatal("the mesh must be built (mesh_build_connectivity)");
+	const int ne = msh->nbelems, nb = msh->nboundaryfaces;
+	double *geom = xmalloc(sizeof(double) * 10 * (size_t)ne);
+	int *nbr = xmalloc(sizeof(int) * 4 * (size_t)ne);
+	int *nbmap = xmalloc(sizeof(int) * 4 * (size_t)ne);
+	double *xnode = xmalloc(sizeof(double) * 30 * (size_t)ne);
+	double *xghost = xmalloc(sizeof(double) * 18 * (size_t)(nb ? nb : 1));
+
+	for (int e = 0; e < ne; e++) {
+		gdn_real codtau[3][3], det;
+		mesh_get_elem_codtau(msh, e, codtau);
+		mesh_get_elem_det(msh, e, &det);
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++)
+				geom[10 * e + 3 * r + c] = codtau[r][c];
+		geom[10 * e + 9] = det;
+		for (int i = 0; i < NQN; i++)
+			for (int d = 0; d < 3; d++)
+				xnode[30 * e + 3 * i + d] =
+					msh->node[3 * msh->elem2qnode[NQN * e + i] + d];
+		for (int f = 0; f < NFA; f++) {
+			const int e2 = msh->elem2elem[NFA * e + f];
+			nbr[4 * e + f] = e2;
+			int code = 0;
+			if (e2 >= 0) {
+				for (int a = 0; a < NQV; a++) {
+					const int q = msh->elem2qnode
+						[NQN * e + t10_get_ref_face2node_entry(f, a)];
+					int j = 0;
+					while (j < NQN && msh->elem2qnode[NQN * e2 + j] != q)
+						j++;
+					if (j == NQN)
+						gdn_fatal("inconsistent face nodes");
+					code |= j << (4 * a);
+				}
+			}
+			nbmap[4 * e + f] = code;
+		}
+	}
+	for (int bf = 0; bf < nb; bf++) {
+		const int F = msh->boundaryface[bf];
+		const int eL = msh->face2elem[E2F * F], fL = msh->face2elem[E2F * F + 1];
+		nbr[4 * eL + fL] = -1 - bf;
+		for (int a = 0; a < NQV; a++) {
+			const int q = msh->elem2qnode[NQN * ne + NQV * bf + a];
+			for (int d = 0; d < 3; d++)
+				xghost[18 * bf + 3 * a + d] = msh->node[3 * q + d];
+		}
+	}
+	out->nelem = ne;
+	out->nbface = nb;
+	out->geom = geom;
+	out->nbr = nbr;
+	out->nbmap = nbmap;
+	out->xnode = xnode;
+	out->xghost = xghost;
+}
+
+void gdn_mesh_desc_release(gdh_mesh_desc *d)
+{
+	free((void *)d->geom);
+	free((void *)d->nbr);
+	free((void *)d->nbmap);
+	free((void *)d->xnode);
+	free((void *)d->xghost);
+	memset(d, 0, sizeof(*d));
+}
+
+static int flux_kind(gdn_get_num_flux f)
+{
+	if (f == maxwell_num_flux_upwind)
+		return GDH_FLUX_MAXWELL_UPWIND;
+	if (f == (gdn_get_num_flux)maxwell_num_flux_boundary_metal)
+		return GDH_FLUX_MAXWELL_METAL;
+	return 0;
+}
+
+void gdn_model_describe(const gdn_model *mdl, int kinetic, gdh_model_desc *out)
+{
+	memset(out, 0, sizeof(*out));
+	if (mdl->nb_w != 6)
+		gdn_fatal("the GPU solver implements Maxwell (nb_w = 6) only");
+	out->flux_kind = flux_kind(mdl->get_num_flux);
+	if (out->flux_kind != GDH_FLUX_MAXWELL_UPWIND)
+		gdn_fatal("get_num_flux must be maxwell_num_flux_upwind");
+	out->bflux_kind = flux_kind((gdn_get_num_flux)mdl->get_num_flux_boundary);
+	if (!out->bflux_kind)
+		gdn_fatal("get_num_flux_boundary must be maxwell_num_flux_upwind "
+			  "or maxwell_num_flux_boundary_metal");
+
+	const gdn_get_imposed_data g = mdl->get_imposed_data;
+	const struct gdn_imposed_params *p = gdn_imposed_params();
+	if (!g)
+		gdn_fatal("get_imposed_data is not set");
+	if (g == maxwell_imposed_macro_cos) {
+		out->imposed_kind = GDH_IMPOSED_MACRO_COS;
+	} else if (g == maxwell_imposed_macro_cavity) {
+		out->imposed_kind = GDH_IMPOSED_CAVITY_TM;
+	} else if (g == maxwell_imposed_stationary) {
+		out->imposed_kind = GDH_IMPOSED_STATIONARY;
+	} else if (g == maxwell_imposed_plane_wave) {
+		out->imposed_kind = GDH_IMPOSED_PLANE_WAVE;
+		for (int d = 0; d < 3; d++) {
+			out->imposed_par[d] = p->pw_k[d];
+			out->imposed_par[3 + d] = p->pw_e[d];
+			out->imposed_par[6 + d] = p->pw_h[d];
+		}
+		out->imposed_par[9] = p->pw_omega;
+		out->imposed_par[10] = p->pw_phase;
+	} else if (g == maxwell_imposed_cavity_mode) {
+		out->imposed_kind = GDH_IMPOSED_CAVITY_MODE;
+		for (int d = 0; d < 3; d++) {
+			out->imposed_par[d] = p->cm_k[d];
+			out->imposed_par[3 + d] = p->cm_a[d];
+		}
+		out->imposed_par[6] = p->cm_omega;
+	} else {
+		out->imposed_kind = GDH_IMPOSED_HOST;
+	}
+
+	out->kinetic_kind = GDH_KIN_NONE;
+	if (kinetic) {
+		if (mdl->get_w == d3q4_get_w && mdl->get_feq == d3q4_get_feq &&
+		    mdl->get_fR == d3q4_get_fR)
+			out->kinetic_kind = GDH_KIN_D3Q4;
+		else if (mdl->get_w == d3q6_get_w && mdl->get_feq == d3q6_get_feq &&
+			 mdl->get_fR == d3q6_get_fR)
+			out->kinetic_kind = GDH_KIN_D3Q6;
+		else
+			gdn_fatal("kinetic callbacks must be the d3q4_* or d3q6_* set");
+		if (mdl->nb_v > GDH_MAX_V || !mdl->vi)
+			gdn_fatal("invalid velocity set");
+		out->nb_v = mdl->nb_v;
+		memcpy(out->vi, mdl->vi, sizeof(double) * 3 * mdl->nb_v);
+		out->lambda = mdl->lambda;
+		out->omega = mdl->omega;
+	}
+}
+
+uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl)
+{
+	const int ne = msh->nbelems, nv = mdl->nb_v;
+	uint8_t *cls = calloc((size_t)ne * NFA * nv, 1);
+	if (!cls)
+		gdn_fatal("out of memory");
+	for (int F = 0; F < msh->nbfaces; F++) {
+		const int ieL = msh->face2elem[E2F * F + 0];
+		const int ifL = msh->face2elem[E2F * F + 1];
+		const int ieR = msh->face2elem[E2F * F + 2];
+		const int ifR = msh->face2elem[E2F * F + 3];
+		/* same computation as src/simulation/simulation.c:511-537 */
+		gdn_real physnode[NNO][3], xref[3] = { 0, 0, 0 }, dtau[3][3],
+			codtau[3][3], vn[3];
+		mesh_get_elem_node(msh, ieL, physnode);
+		gdn_tetra_ref2phy(physnode, xref, NULL, ifL, NULL, dtau, codtau,
+				  NULL, vn);
+		const gdn_real cr = ifL == 3 ? sqrt(3) : 1;
+		for (int d = 0; d < 3; d++)
+			vn[d] *= cr;
+		for (int v = 0; v < nv; v++) {
+			const gdn_real *vi = mdl->vi + 3 * v;
+			const gdn_real s = vi[0] * vn[0] + vi[1] * vn[1] + vi[2] * vn[2];
+			uint8_t cL = 0, cR = 0;
+			if (s > 0) {
+				cL = 1;
+				cR = 2;
+			} else if (s < 0) {
+				cL = 2;
+				cR = 1;
+			}
+			cls[((size_t)ieL * NFA + ifL) * nv + v] = cL;
+			if (ieR >= 0)
+				cls[((size_t)ieR * NFA + ifR) * nv + v] = cR;
+		}
+	}
+	return cls;
+}
+
+double *gdn_eval_imposed_elements(const gdn_simulation *simu, double t)
+{
+	const gdn_mesh *msh = simu->tt;
+	const int nw = simu->mdl->nb_w;
+	double *w = xmalloc(sizeof(double) * (size_t)NQN * msh->nbelems * nw);
+	for (int ipg = 0; ipg < NQN * msh->nbelems; ipg++)
+		simu->mdl->get_imposed_data(msh->node + 3 * msh->elem2qnode[ipg], t,
+					    w + (size_t)ipg * nw);
+	return w;
+}
+
+double *gdn_eval_imposed_ghosts(const gdn_simulation *simu, double t)
+{
+	const gdn_mesh *msh = simu->tt;
+	const int nw = simu->mdl->nb_w, nb = msh->nboundaryfaces;
+	double *w = xmalloc(sizeof(double) * (size_t)NQV * (nb ? nb : 1) * nw);
+	for (int k = 0; k < NQV * nb; k++) {
+		const int q = msh->elem2qnode[NQN * msh->nbelems + k];
+		simu->mdl->get_imposed_data(msh->node + 3 * q, t,
+					    w + (size_t)k * nw);
+	}
+	return w;
+}
+
+gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md)
+{
+	if (gdh_device_count() < 1)
+		gdn_fatal("no HIP device is visible; the GPU solver cannot run");
+	gdn_model_describe(simu->mdl, simu->use_kinetic_scheme, md);
+	gdh_mesh_desc d;
+	gdn_mesh_describe(simu->tt, &d);
+	gdh_ctx *ctx = gdh_create(&d, md, simu->use_kinetic_scheme ? 1 : 0);
+	gdn_mesh_desc_release(&d);
+	gdh_upload_state(ctx, simu->wn);
+	return ctx;
+}

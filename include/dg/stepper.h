@@ -1,0 +1,47 @@
+/*
+ * Step-by-step access to the GPU solvers. No counterpart in the reference,
+ * whose dg_solve_rk3 / dg_solve_relaxation (src/dg/explicit.c:39-117,
+ * src/dg/relaxation.c:129-185) only run a whole time loop; those two entry
+ * points are implemented on top of this interface, and benchmark or coupling
+ * drivers can use it to advance, time and inspect the device state.
+ *
+ * Usage: st = dg_stepper_create(simu) (set-up = the reference's assembly);
+ * dg_stepper_init(st) (initial condition at t = 0); dg_stepper_advance(st, k)
+ * (k full time steps, queued asynchronously on the device stream);
+ * dg_stepper_finish(st) (final ghost update for the kinetic scheme, copy
+ * into simu->wn, simu->t set); dg_stepper_free(st).
+ */
+#ifndef DG_STEPPER_H
+#define DG_STEPPER_H
+
+#include <simulation/simulation.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gdn_stepper gdn_stepper;
+
+gdn_stepper *dg_stepper_create(gdn_simulation *simu);
+void dg_stepper_init(gdn_stepper *st);
+void dg_stepper_advance(gdn_stepper *st, int nsteps);
+gdn_real dg_stepper_time(const gdn_stepper *st);
+int dg_stepper_iterations(const gdn_stepper *st);
+void dg_stepper_synchronize(gdn_stepper *st);
+/* L2 error against get_imposed_data at the current time (device reduction) */
+gdn_real dg_stepper_error_l2(gdn_stepper *st);
+void dg_stepper_finish(gdn_stepper *st);
+void dg_stepper_free(gdn_stepper *st);
+
+/* Device-event timing of the main kernels since the last reset, seconds:
+ * out[0] transport sweep (kinetic) or RK3 stage kernel, out[1] relaxation
+ * kernel, out[2] ghost update, out[3] number of sweep / stage launches. */
+void dg_stepper_profile(gdn_stepper *st, int enable, double out[4]);
+
+/* Number of upwind levels of the worst velocity (kinetic scheme, else 0). */
+int dg_stepper_levels(const gdn_stepper *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

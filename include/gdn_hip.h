@@ -1,0 +1,144 @@
+/*
+ * gdn_hip.h -- C ABI of the GPU layer (HIP, gfx950).
+ *
+ * This is the thin boundary between the C99 host code that implements the
+ * reference API (dg_solve_rk3, dg_solve_relaxation, simulation_error_l2, ...)
+ * and the hand-written HIP kernels. Everything crossing it is a plain C type:
+ * pointers to host arrays, sizes and small POD descriptors. No reference
+ * function maps to it one-to-one; it is what src/dg/explicit.c:39-117,
+ * src/dg/relaxation.c:93-185, src/dg/dg.c:32-147 and
+ * src/simulation/simulation.c:606-666 are rebuilt on.
+ *
+ * Errors: every call checks the HIP status and aborts with a message naming
+ * the failing call (the reference aborts through assert()).
+ */
+#ifndef GDN_HIP_H
+#define GDN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ kinds */
+enum gdh_flux_kind {
+	GDH_FLUX_MAXWELL_UPWIND = 1, /* maxwell_num_flux_upwind */
+	GDH_FLUX_MAXWELL_METAL = 2, /* maxwell_num_flux_boundary_metal */
+};
+
+enum gdh_imposed_kind {
+	GDH_IMPOSED_HOST = 0, /* unknown callback: ghosts come from the host */
+	GDH_IMPOSED_MACRO_COS = 1, /* maxwell_imposed_macro_cos */
+	GDH_IMPOSED_CAVITY_TM = 2, /* maxwell_imposed_macro_cavity */
+	GDH_IMPOSED_STATIONARY = 3, /* maxwell_imposed_stationary */
+	GDH_IMPOSED_PLANE_WAVE = 4, /* maxwell_imposed_plane_wave */
+	GDH_IMPOSED_CAVITY_MODE = 5, /* maxwell_imposed_cavity_mode */
+};
+
+enum gdh_kinetic_kind {
+	GDH_KIN_NONE = 0,
+	GDH_KIN_D3Q4 = 1, /* d3q4_get_w / d3q4_get_feq / d3q4_get_fR */
+	GDH_KIN_D3Q6 = 2, /* d3q6_get_w / d3q6_get_feq / d3q6_get_fR */
+};
+
+#define GDH_MAX_V 8
+#define GDH_IMPOSED_NPAR 16
+
+/* Element geometry and neighbourhood, element-major, built by the host. */
+typedef struct gdh_mesh_desc {
+	int nelem; /* number of tetrahedra */
+	int nbface; /* number of boundary faces (ghost blocks of 6 DoFs) */
+	/* nelem * 10: codtau[3][3] row-major then det (src/mesh/mesh.c:81-102) */
+	const double *geom;
+	/* nelem * 4: neighbour element across local face f, or -1 - bf when
+	 * local face f is boundary face bf (the bf-th entry of boundaryface) */
+	const int *nbr;
+	/* nelem * 4: for an interior face, 4 bits per own face node a (a = 0..5,
+	 * own face order t10 face2node[f]) giving the neighbour's local T10 node
+	 * that carries the same point */
+	const int *nbmap;
+	/* nelem * 10 * 3: coordinates of the T10 nodes, element by element */
+	const double *xnode;
+	/* nbface * 6 * 3: coordinates of the ghost DoFs (the L node of each face
+	 * node, src/dg/dg.c:87-93) */
+	const double *xghost;
+} gdh_mesh_desc;
+
+/* Physics recognised from the gdn_model callbacks. */
+typedef struct gdh_model_desc {
+	int flux_kind; /* interior faces */
+	int bflux_kind; /* boundary faces */
+	int imposed_kind;
+	double imposed_par[GDH_IMPOSED_NPAR];
+	int kinetic_kind;
+	int nb_v;
+	double vi[3 * GDH_MAX_V];
+	double lambda;
+	double omega;
+} gdh_model_desc;
+
+typedef struct gdh_ctx gdh_ctx;
+
+/* Number of visible GPUs (0 when none); does not create a context. */
+int gdh_device_count(void);
+
+/* Upload the mesh and the model; scheme 0 = RK3 (macro), 1 = kinetic. */
+gdh_ctx *gdh_create(const gdh_mesh_desc *mesh, const gdh_model_desc *model,
+		    int scheme);
+void gdh_destroy(gdh_ctx *ctx);
+
+/* State transfer in the reference layout of simu->wn (m * neq doubles). */
+void gdh_upload_state(gdh_ctx *ctx, const double *wn);
+void gdh_download_state(gdh_ctx *ctx, double *wn);
+
+/* Initial condition at every element DoF (src/dg/dg.c:32-59). For
+ * GDH_IMPOSED_HOST, wexact holds get_imposed_data(x, 0) for the 10 nelem
+ * element nodes (nb_w values each); else it may be NULL. */
+void gdh_init_solution(gdh_ctx *ctx, const double *wexact);
+
+/* Ghost DoFs at time t (src/dg/dg.c:61-107). For GDH_IMPOSED_HOST, wghost
+ * holds get_imposed_data(x, t) at the 6 nbface ghost nodes. */
+void gdh_update_boundary(gdh_ctx *ctx, double t, const double *wghost);
+
+/* One RK3 stage (0, 1, 2) of src/dg/explicit.c:76-96 on the device state. */
+void gdh_rk3_stage(gdh_ctx *ctx, int stage, double dt);
+
+/* Kinetic solver set-up for time step dt (src/dg/relaxation.c:24-91):
+ * upwind classification, level sets, block factors. Returns the number of
+ * upwind levels of the worst velocity; aborts on a cyclic upwind graph. */
+int gdh_relax_setup(gdh_ctx *ctx, double dt, const uint8_t *upwind_class);
+
+/* Transport (src/dg/relaxation.c:93-127) + swap + relaxation
+ * (src/dg/dg.c:109-147) on the device state. */
+void gdh_relax_step(gdh_ctx *ctx);
+
+/* Node-local relaxation only (src/dg/dg.c:109-147). */
+void gdh_apply_relaxation(gdh_ctx *ctx);
+
+/* src/simulation/simulation.c:606-666 on the device state at time t. For
+ * GDH_IMPOSED_HOST, wexact holds the exact solution at the element nodes. */
+double gdh_error_l2(gdh_ctx *ctx, double t, const double *wexact);
+
+/* Event timing of the main kernels: when enabled, each sweep (kinetic) or
+ * stage kernel (RK3), relaxation kernel and ghost update is bracketed by HIP
+ * events recorded on the context stream (no host wait in between). This call
+ * waits for the stream, and out (may be NULL) receives the seconds of device
+ * time accumulated since the previous call: [0] sweep / stage,
+ * [1] relaxation, [2] ghost update, [3] sweep or stage launches; the
+ * counters are then reset. */
+void gdh_profile(gdh_ctx *ctx, int enable, double *out);
+
+/* Wait for all queued work. */
+void gdh_synchronize(gdh_ctx *ctx);
+
+/* Device time between gdh_timer_start and gdh_timer_stop, seconds (HIP
+ * events on the context stream). */
+void gdh_timer_start(gdh_ctx *ctx);
+double gdh_timer_stop(gdh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,45 @@
+"""Shared helpers of the test suite: synthetic meshes and comparisons."""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+import gdon3d as G
+import oracle as O
+
+_TMP = tempfile.mkdtemp(prefix="gdon3d_tests_")
+
+
+def kuhn_msh(n, jitter=0.0, seed=0):
+    """gmsh v2.2 file of the Kuhn n^3 x 6 cube (written once per process)."""
+    path = os.path.join(_TMP, f"kuhn_{n}_{jitter}_{seed}.msh")
+    if not os.path.exists(path):
+        rc = G.load_library().mesh_kuhn_cube_write_msh22(path.encode(), n, jitter, seed)
+        assert rc == 0
+    return path
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def plane_wave_par(k=(2 * np.pi, 0, 0), e0=(0, 1, 0), phase=0.0):
+    """Parameter block of the plane-wave exact solution (as gdn_hip.h)."""
+    k = np.asarray(k, dtype=np.float64)
+    e0 = np.asarray(e0, dtype=np.float64)
+    h0 = np.cross(k / np.linalg.norm(k), e0)
+    par = np.zeros(16)
+    par[0:3], par[3:6], par[6:9] = k, e0, h0
+    par[9], par[10] = np.linalg.norm(k), phase
+    return par
+
+
+def cavity_par(lmn=(1, 1, 1), a=(1, -1, 0)):
+    k = np.pi * np.asarray(lmn, dtype=np.float64)
+    par = np.zeros(16)
+    par[0:3], par[3:6] = k, a
+    par[6] = np.pi * np.sqrt(float(np.dot(lmn, lmn)))
+    return par

@@ -1,0 +1,62 @@
+"""GPU parity of the CFL-less relaxation solve (dg_solve_relaxation through
+the C ABI: level-scheduled block sweep + node relaxation) against the
+oracle's assembled theta-scheme with sparse LU (src/dg/relaxation.c:24-185).
+
+Tolerance: relative L2 <= 1e-12 on the full micro state wn (FP64)."""
+import numpy as np
+import pytest
+
+import gdon3d as G
+import oracle as O
+from helpers import kuhn_msh, plane_wave_par, cavity_par, rel_l2
+
+TOL = 1e-12
+
+
+def run_pair(path, imposed, kind, par, kin, tend, cfl):
+    mesh = G.Mesh(path=path)
+    model = G.Model(imposed, kinetic=kin)
+    sim = G.Simulation(mesh, model)
+    sim.set_time(tend, 0.0, cfl)
+    sim.solve()
+    g = dict(wn=sim.wn(), err=sim.error_l2(), itermax=sim.s.itermax, t=sim.s.t)
+    sim.free()
+    om = O.OMesh(path)
+    s = O.OSim(om, 1, kind, par, kin=O.D3Q4 if kin == "d3q4" else O.D3Q6)
+    s.set_time(tend, 0.0, cfl)
+    s.solve()
+    o = dict(wn=s.wn(), err=s.error_l2(), itermax=s.itermax, t=s.t)
+    s.free()
+    om.free()
+    return g, o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,jitter,tend,cfl", [(3, 0.0, 0.3, 5.0), (4, 0.15, 0.3, 5.0),
+                                               (6, 0.0, 0.5, 5.0)])
+def test_relax_cavity_d3q4(n, jitter, tend, cfl):
+    print(f"relax cavity n={n} jitter={jitter}", flush=True)
+    path = kuhn_msh(n, jitter, 23)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    g, o = run_pair(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", tend, cfl)
+    assert g["itermax"] == o["itermax"]
+    assert rel_l2(g["wn"], o["wn"]) <= TOL
+    assert abs(g["err"] - o["err"]) <= 1e-10 * o["err"]
+
+
+@pytest.mark.gpu
+def test_relax_plane_wave_d3q4_jittered():
+    path = kuhn_msh(4, 0.2, 7)
+    k = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
+    G.set_plane_wave(k, (0, 0, 1))
+    g, o = run_pair(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(k, (0, 0, 1)),
+                    "d3q4", 0.3, 2.0)
+    assert rel_l2(g["wn"], o["wn"]) <= TOL
+
+
+@pytest.mark.gpu
+def test_relax_d3q6():
+    path = kuhn_msh(3, 0.1, 9)
+    G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
+    g, o = run_pair(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(), "d3q6", 0.25, 3.0)
+    assert rel_l2(g["wn"], o["wn"]) <= TOL
