@@ -710,7 +710,9 @@ __global__ void k_block_inverse(const int2 *__restrict__ items, int nitems,
 			out[10 * i + j] = A[i][10 + j];
 }
 
-extern "C" int gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls)
+extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
+				const int *items_ve, const int *level_ptr,
+				int nlevels)
 {
 	if (!c->scheme) {
 		fprintf(stderr, "gdon3d-hip: relaxation set-up on an RK3 context\n");
@@ -718,68 +720,23 @@ extern "C" int gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls)
 	}
 	const int ne = c->ne, nv = c->kp.nv;
 	c->dt = dt;
-	/* host copy of the neighbourhood for the dependency graph */
-	std::vector<int4> nbr(ne);
-	HIPCHECK(hipMemcpy(nbr.data(), c->nbr, sizeof(int4) * ne,
-			   hipMemcpyDeviceToHost));
-	auto nb_of = [&](int e, int f) {
-		const int4 q = nbr[e];
-		return f == 0 ? q.x : f == 1 ? q.y : f == 2 ? q.z : q.w;
-	};
-	/* per velocity: longest-path levels by Kahn's algorithm */
-	std::vector<std::vector<int>> buckets;
-	std::vector<int> level(ne), indeg(ne), queue(ne);
-	int maxlev = 0;
-	for (int v = 0; v < nv; v++) {
-		for (int e = 0; e < ne; e++) {
-			int d = 0;
-			for (int f = 0; f < 4; f++)
-				d += cls[(4L * e + f) * nv + v] == 2 && nb_of(e, f) >= 0;
-			indeg[e] = d;
-			level[e] = 0;
-		}
-		int head = 0, tail = 0;
-		for (int e = 0; e < ne; e++)
-			if (!indeg[e])
-				queue[tail++] = e;
-		while (head < tail) {
-			const int e = queue[head++];
-			for (int f = 0; f < 4; f++) {
-				const int nb = nb_of(e, f);
-				if (nb < 0 || cls[(4L * e + f) * nv + v] != 1)
-					continue;
-				level[nb] = std::max(level[nb], level[e] + 1);
-				if (--indeg[nb] == 0)
-					queue[tail++] = nb;
-			}
-		}
-		if (tail != ne) {
-			fprintf(stderr,
-				"gdon3d-hip: the upwind graph of velocity %d has a "
-				"cycle (%d of %d elements ordered); the block sweep "
-				"needs an acyclic graph\n",
-				v, tail, ne);
+	c->nlevels = nlevels;
+	c->level_ptr.assign(level_ptr, level_ptr + nlevels + 1);
+	c->nitems = level_ptr[nlevels];
+	if (c->nitems != nv * ne) {
+		fprintf(stderr, "gdon3d-hip: sweep schedule has %d items, expected %d\n",
+			c->nitems, nv * ne);
+		abort();
+	}
+	std::vector<int2> items(c->nitems);
+	for (int k = 0; k < c->nitems; k++) {
+		const int ve = items_ve[k];
+		if (ve < 0 || ve >= nv * ne) {
+			fprintf(stderr, "gdon3d-hip: invalid sweep item %d\n", ve);
 			abort();
 		}
-		for (int e = 0; e < ne; e++) {
-			if (level[e] >= (int)buckets.size())
-				buckets.resize(level[e] + 1);
-			buckets[level[e]].push_back(v * ne + e);
-			maxlev = std::max(maxlev, level[e]);
-		}
+		items[k] = make_int2(ve / ne, ve % ne);
 	}
-	c->nlevels = (int)buckets.size();
-	c->level_ptr.assign(c->nlevels + 1, 0);
-	std::vector<int2> items;
-	items.reserve((size_t)nv * ne);
-	for (int L = 0; L < c->nlevels; L++) {
-		c->level_ptr[L] = (int)items.size();
-		for (int ve : buckets[L])
-			items.push_back(make_int2(ve / ne, ve % ne));
-	}
-	c->level_ptr[c->nlevels] = (int)items.size();
-	c->nitems = (int)items.size();
-
 	c->items = (int2 *)dmalloc(sizeof(int2) * items.size());
 	HIPCHECK(hipMemcpy(c->items, items.data(), sizeof(int2) * items.size(),
 			   hipMemcpyHostToDevice));
@@ -790,7 +747,6 @@ extern "C" int gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls)
 	k_block_inverse<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
 		c->items, c->nitems, c->geom, c->cls, c->kp, dt, c->dinv);
 	HIPCHECK(hipGetLastError());
-	return maxlev + 1;
 }
 
 /*

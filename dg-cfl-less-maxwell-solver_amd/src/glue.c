@@ -211,6 +211,76 @@ uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl)
 	return cls;
 }
 
+int gdn_sweep_schedule(int nelem, int nv, const int *nbr, const uint8_t *cls,
+		       int *items, int *level_ptr, int *nlevels)
+{
+	int *indeg = malloc(sizeof(int) * (size_t)(nelem ? nelem : 1));
+	int *queue = malloc(sizeof(int) * (size_t)(nelem ? nelem : 1));
+	int *level = malloc(sizeof(int) * (size_t)nv * (nelem ? nelem : 1));
+	if (!indeg || !queue || !level)
+		gdn_fatal("out of memory");
+	int maxlev = -1, rc = 0;
+	for (int v = 0; v < nv && !rc; v++) {
+		int *lev = level + (size_t)v * nelem;
+		for (int e = 0; e < nelem; e++) {
+			int d = 0;
+			for (int f = 0; f < 4; f++)
+				d += cls[((size_t)e * 4 + f) * nv + v] == 2 &&
+				     nbr[4 * e + f] >= 0;
+			indeg[e] = d;
+			lev[e] = 0;
+		}
+		int head = 0, tail = 0;
+		for (int e = 0; e < nelem; e++)
+			if (!indeg[e])
+				queue[tail++] = e;
+		while (head < tail) {
+			const int e = queue[head++];
+			for (int f = 0; f < 4; f++) {
+				const int nb = nbr[4 * e + f];
+				if (nb < 0 || cls[((size_t)e * 4 + f) * nv + v] != 1)
+					continue;
+				if (lev[e] + 1 > lev[nb])
+					lev[nb] = lev[e] + 1;
+				if (--indeg[nb] == 0)
+					queue[tail++] = nb;
+			}
+		}
+		if (tail != nelem) {
+			rc = -1 - v;
+			break;
+		}
+		for (int e = 0; e < nelem; e++)
+			if (lev[e] > maxlev)
+				maxlev = lev[e];
+	}
+	if (!rc) {
+		/* counting sort by level; (v, e) increasing inside a level */
+		const int nl = maxlev + 1;
+		for (int L = 0; L <= nl; L++)
+			level_ptr[L] = 0;
+		for (int v = 0; v < nv; v++)
+			for (int e = 0; e < nelem; e++)
+				level_ptr[level[(size_t)v * nelem + e] + 1]++;
+		for (int L = 0; L < nl; L++)
+			level_ptr[L + 1] += level_ptr[L];
+		int *pos = malloc(sizeof(int) * (size_t)(nl ? nl : 1));
+		if (!pos)
+			gdn_fatal("out of memory");
+		for (int L = 0; L < nl; L++)
+			pos[L] = level_ptr[L];
+		for (int v = 0; v < nv; v++)
+			for (int e = 0; e < nelem; e++)
+				items[pos[level[(size_t)v * nelem + e]]++] = v * nelem + e;
+		free(pos);
+		*nlevels = nl;
+	}
+	free(indeg);
+	free(queue);
+	free(level);
+	return rc;
+}
+
 double *gdn_eval_imposed_elements(const gdn_simulation *simu, double t)
 {
 	const gdn_mesh *msh = simu->tt;

@@ -26,6 +26,15 @@ void gdn_mesh_desc_release(gdh_mesh_desc *d);
  * (src/simulation/simulation.c:127-136,511-537). Caller frees. */
 uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl);
 
+/* Level-ordered work list of the upwind transport sweep (longest-path
+ * levels of each velocity's upwind graph, Kahn's algorithm). nbr is the
+ * 4-per-element neighbour array of gdh_mesh_desc, cls the classes above.
+ * Fills items[nv * nelem] with v * nelem + e sorted by (level, v, e) and
+ * level_ptr[nlevels + 1]; returns 0, or -1 - v when the graph of velocity v
+ * has a cycle (then items is unspecified). */
+int gdn_sweep_schedule(int nelem, int nv, const int *nbr, const uint8_t *cls,
+		       int *items, int *level_ptr, int *nlevels);
+
 /* A device context for the simulation with its state uploaded (and the
  * imposed-data arrays of GDH_IMPOSED_HOST helpers). */
 gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md);

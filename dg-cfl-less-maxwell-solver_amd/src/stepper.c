@@ -5,6 +5,7 @@
  * relaxation; final ghosts at t), split into create / init / advance /
  * finish so that drivers can time or inspect single steps.
  */
+#include <stdio.h>
 #include <stdlib.h>
 
 #include <dg/stepper.h>
@@ -37,8 +38,29 @@ gdn_stepper *dg_stepper_create(gdn_simulation *simu)
 	st->simu = simu;
 	st->ctx = gdn_ctx_open(simu, &st->md);
 	if (simu->use_kinetic_scheme) {
-		uint8_t *cls = gdn_upwind_classes(simu->tt, simu->mdl);
-		st->levels = gdh_relax_setup(st->ctx, simu->dt, cls);
+		const gdn_mesh *msh = simu->tt;
+		const int ne = msh->nbelems, nv = simu->mdl->nb_v;
+		uint8_t *cls = gdn_upwind_classes(msh, simu->mdl);
+		gdh_mesh_desc d;
+		gdn_mesh_describe(msh, &d);
+		int *items = malloc(sizeof(int) * (size_t)nv * ne);
+		int *lptr = malloc(sizeof(int) * ((size_t)nv * ne + 1));
+		if (!items || !lptr)
+			gdn_fatal("out of memory");
+		int nl = 0;
+		const int rc = gdn_sweep_schedule(ne, nv, d.nbr, cls, items, lptr, &nl);
+		gdn_mesh_desc_release(&d);
+		if (rc) {
+			fprintf(stderr,
+				"gdon3d: the upwind graph of velocity %d has a cycle; "
+				"the block-triangular sweep needs an acyclic graph\n",
+				-1 - rc);
+			abort();
+		}
+		st->levels = nl;
+		gdh_relax_setup(st->ctx, simu->dt, cls, items, lptr, nl);
+		free(items);
+		free(lptr);
 		free(cls);
 	}
 	gdh_synchronize(st->ctx);

@@ -105,10 +105,14 @@ void gdh_update_boundary(gdh_ctx *ctx, double t, const double *wghost);
 /* One RK3 stage (0, 1, 2) of src/dg/explicit.c:76-96 on the device state. */
 void gdh_rk3_stage(gdh_ctx *ctx, int stage, double dt);
 
-/* Kinetic solver set-up for time step dt (src/dg/relaxation.c:24-91):
- * upwind classification, level sets, block factors. Returns the number of
- * upwind levels of the worst velocity; aborts on a cyclic upwind graph. */
-int gdh_relax_setup(gdh_ctx *ctx, double dt, const uint8_t *upwind_class);
+/* Kinetic solver set-up for time step dt (src/dg/relaxation.c:24-91).
+ * upwind_class: nelem * 4 * nb_v bytes, 0 no flux / 1 outflow / 2 inflow
+ * per (element, local face, velocity); items (nb_v * nelem entries
+ * v * nelem + e) and level_ptr (nlevels + 1 offsets): the sweep schedule in
+ * upwind level order, every element after all its inflow neighbours. Builds
+ * the per-item 10x10 block factors on the device. */
+void gdh_relax_setup(gdh_ctx *ctx, double dt, const uint8_t *upwind_class,
+		     const int *items, const int *level_ptr, int nlevels);
 
 /* Transport (src/dg/relaxation.c:93-127) + swap + relaxation
  * (src/dg/dg.c:109-147) on the device state. */

@@ -1739,6 +1739,37 @@ void orc_block_stats(void *h, int iv, int *out)
 	out[1] = mx;
 }
 
+/* upwind dependencies read off the assembled M1 of velocity iv: for each
+ * element, the other elements its rows reference (-2 for ghost DoFs),
+ * 4 slots per element, -1 padded */
+void orc_m1_dependencies(void *h, int iv, int *out)
+{
+	osim *s = h;
+	assemble(s);
+	const csr *A = &s->M1[iv];
+	const int ne = s->m->nelem;
+	for (int e = 0; e < ne; e++) {
+		int k = 0, *o = out + 4 * e;
+		o[0] = o[1] = o[2] = o[3] = -1;
+		for (int r = NQN * e; r < NQN * e + NQN; r++)
+			for (int64_t q = A->p[r]; q < A->p[r + 1]; q++) {
+				if (A->x[q] == 0.0)
+					continue;
+				const int c = A->j[q];
+				const int ce = c < NQN * ne ? c / NQN : -2;
+				if (ce == e)
+					continue;
+				int seen = 0;
+				for (int u = 0; u < k; u++)
+					seen |= o[u] == ce && ce != -2;
+				if (seen || (ce == -2 && k && o[k - 1] == -2))
+					continue;
+				if (k < 4)
+					o[k++] = ce;
+			}
+	}
+}
+
 void orc_relax(void *h, int with_ghosts)
 {
 	osim *s = h;

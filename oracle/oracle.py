@@ -52,6 +52,7 @@ def lib():
             "orc_transport_residual": (C.c_double, [C.c_void_p, C.c_int, dp, dp]),
             "orc_block_stats": (None, [C.c_void_p, C.c_int, ip]),
             "orc_relax": (None, [C.c_void_p, C.c_int]),
+            "orc_m1_dependencies": (None, [C.c_void_p, C.c_int, ip]),
             "orc_update_boundary": (None, [C.c_void_p, C.c_double]),
             "orc_init_solution": (None, [C.c_void_p]),
             "orc_error_l2": (C.c_double, [C.c_void_p]),
@@ -170,6 +171,11 @@ class OSim:
         out = (C.c_int * 2)()
         lib().orc_block_stats(self.h, iv, out)
         return list(out)
+
+    def m1_dependencies(self, iv):
+        out = np.zeros(4 * self.mesh.nelem, np.int32)
+        lib().orc_m1_dependencies(self.h, iv, out.ctypes.data_as(ip))
+        return out.reshape(-1, 4)
 
     def relax(self, with_ghosts=True):
         lib().orc_relax(self.h, int(with_ghosts))

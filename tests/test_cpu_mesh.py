@@ -1,0 +1,187 @@
+"""Mesh input and connectivity: the library's hash-based builder, the oracle's
+restatement of the reference's qsort-based one, and the reference's own
+src/mesh/mesh.c (built in oracle/_ref) on synthetic Kuhn meshes and on the
+reference's shipped gmsh files."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import gdon3d as G
+import oracle as O
+from helpers import kuhn_msh
+
+SHIPPED = ["/root/reference/data/mesh/t4_cube_8.msh",
+           "/root/reference/data/mesh/t4_cube_16.msh",
+           "/root/reference/data/mesh/t4_cube_8_torus_128.msh"]
+KUHN = [(1, 0.0, 0), (2, 0.0, 0), (3, 0.1, 17), (4, 0.2, 23), (5, 0.0, 0)]
+
+
+class _RefMesh(G.Mesh):
+    def __init__(self, path):
+        R = O.ref()
+        R.mesh_read_from_msh22_file.argtypes = [C.POINTER(G.GdnMesh), C.c_char_p]
+        R.mesh_build_connectivity.argtypes = [C.POINTER(G.GdnMesh)]
+        self.s = G.GdnMesh()
+        R.mesh_read_from_msh22_file(C.byref(self.s), str(path).encode())
+        R.mesh_build_connectivity(C.byref(self.s))
+
+
+def _paths():
+    out = [kuhn_msh(*k) for k in KUHN]
+    out += [p for p in SHIPPED if os.path.exists(p)]
+    return out
+
+
+@pytest.mark.parametrize("path", _paths())
+def test_connectivity_library_equals_oracle(path):
+    pm = G.Mesh(path=path)
+    om = O.OMesh(path)
+    pa, oa = pm.arrays(), om.arrays()
+    for k in pa:
+        assert np.array_equal(pa[k], oa[k]), k
+    lib = G.load_library()
+    assert lib.mesh_get_t4_hmin(C.byref(pm.s)) == om.hmin()
+    pm.free()
+    om.free()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
+@pytest.mark.parametrize("path", _paths())
+def test_connectivity_equals_reference_build(path):
+    pm, rm = G.Mesh(path=path), _RefMesh(path)
+    pa, ra = pm.arrays(), rm.arrays()
+    for k in pa:
+        assert np.array_equal(pa[k], ra[k]), k
+    R = O.ref()
+    R.mesh_get_t4_hmin.argtypes = [C.POINTER(G.GdnMesh)]
+    R.mesh_get_t4_hmin.restype = C.c_double
+    lib = G.load_library()
+    assert lib.mesh_get_t4_hmin(C.byref(pm.s)) == R.mesh_get_t4_hmin(C.byref(rm.s))
+    # element geometry and face normals, bit for bit
+    for name in ("mesh_get_elem_codtau", "mesh_get_elem_det", "mesh_get_face_vn"):
+        getattr(R, name).argtypes = [C.POINTER(G.GdnMesh), C.c_int, C.POINTER(C.c_double)]
+    a, b = np.zeros(9), np.zeros(9)
+    for e in range(0, pm.nelem, max(1, pm.nelem // 50)):
+        lib.mesh_get_elem_codtau(C.byref(pm.s), e, G.dptr(a))
+        R.mesh_get_elem_codtau(C.byref(rm.s), e, G.dptr(b))
+        assert np.array_equal(a, b)
+        lib.mesh_get_elem_det(C.byref(pm.s), e, G.dptr(a))
+        R.mesh_get_elem_det(C.byref(rm.s), e, G.dptr(b))
+        assert a[0] == b[0]
+    for f in range(0, pm.s.nbfaces, max(1, pm.s.nbfaces // 50)):
+        lib.mesh_get_face_vn(C.byref(pm.s), f, G.dptr(a))
+        R.mesh_get_face_vn(C.byref(rm.s), f, G.dptr(b))
+        assert np.array_equal(a[:3], b[:3])
+    pm.free()
+
+
+@pytest.mark.parametrize("n", [1, 2, 4])
+def test_kuhn_counts_and_geometry(n):
+    m = G.Mesh(kuhn=(n, 0.0, 0))
+    s = m.s
+    assert s.nbelems == 6 * n ** 3
+    assert s.nbqnodes_in == (2 * n + 1) ** 3
+    assert s.nbfaces == 12 * n ** 3 + 6 * n ** 2
+    assert s.nboundaryfaces == 12 * n ** 2
+    lib = G.load_library()
+    h = 1.0 / n
+    # min volume/surface of a Kuhn tetrahedron: h / (6 (1 + sqrt 2))
+    assert abs(lib.mesh_get_t4_hmin(C.byref(s)) - h / (6 * (1 + np.sqrt(2)))) < 1e-15
+    node = m.arr("node", 3 * s.nbqnodes_in, np.float64).reshape(-1, 3)
+    e2q = m.arr("elem2qnode", 10 * s.nbelems).reshape(-1, 10)
+    cod, det = np.zeros(9), np.zeros(1)
+    for e in range(s.nbelems):
+        x = node[e2q[e, :4]]
+        J = (x[1:] - x[0]).T  # columns x1-x0, x2-x0, x3-x0
+        lib.mesh_get_elem_det(C.byref(s), e, G.dptr(det))
+        lib.mesh_get_elem_codtau(C.byref(s), e, G.dptr(cod))
+        assert det[0] > 0
+        assert abs(det[0] - np.linalg.det(J)) < 1e-15
+        assert abs(det[0] - h ** 3) < 1e-15  # volume h^3/6
+        assert np.allclose(cod.reshape(3, 3), det[0] * np.linalg.inv(J).T, atol=1e-15)
+        # T10 edge nodes are midpoints
+        for k, (a, b) in enumerate([(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]):
+            assert np.allclose(node[e2q[e, 4 + k]], 0.5 * (x[a] + x[b]), atol=1e-16)
+    m.free()
+
+
+def test_face_normals_are_twice_the_area():
+    """|vn| of the L element's face = 2 * area (the area-1/2 reference face),
+    vn points out of L, and the whole boundary closes (sum of vn = 0)."""
+    m = G.Mesh(kuhn=(2, 0.2, 3))
+    lib = G.load_library()
+    s = m.s
+    node = m.arr("node", 3 * s.nbqnodes_in, np.float64).reshape(-1, 3)
+    e2q = m.arr("elem2qnode", 10 * s.nbelems).reshape(-1, 10)
+    f2e = m.arr("face2elem", 4 * s.nbfaces).reshape(-1, 4)
+    t4face = [(0, 3, 2), (0, 1, 3), (0, 2, 1), (1, 2, 3)]
+    vn = np.zeros(3)
+    total = np.zeros(3)
+    for F in range(s.nbfaces):
+        lib.mesh_get_face_vn(C.byref(s), F, G.dptr(vn))
+        eL, fL = f2e[F, 0], f2e[F, 1]
+        if F == 3 or fL == 3:
+            # mesh_get_face_vn applies sqrt(3) on global face 3 only (as the
+            # reference); undo / redo it to get the geometric normal
+            scale = (np.sqrt(3) if fL == 3 else 1.0) / (np.sqrt(3) if F == 3 else 1.0)
+            vn = vn * scale
+        p = node[e2q[eL, list(t4face[fL])]]
+        area = 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+        assert abs(np.linalg.norm(vn) - 2 * area) < 1e-14
+        centroid = node[e2q[eL, :4]].mean(axis=0)
+        assert np.dot(vn, p.mean(axis=0) - centroid) > 0
+        if f2e[F, 2] < 0:
+            total += vn
+    assert np.abs(total).max() < 1e-14
+    m.free()
+
+
+def _splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
+    return state, z ^ (z >> 31)
+
+
+def test_kuhn_jitter_prng():
+    """Interior vertices move by jitter*h*(2u-1), u from SplitMix64(seed);
+    boundary vertices stay on the cube faces."""
+    n, jit, seed = 4, 0.2, 23
+    m = G.GdnMesh()
+    G.load_library().mesh_kuhn_cube(C.byref(m), n, jit, seed)
+    x = np.ctypeslib.as_array(m.node, shape=(3 * m.nbnodes,)).reshape(-1, 3).copy()
+    st = seed
+    h = 1.0 / n
+    for k in range(n + 1):
+        for j in range(n + 1):
+            for i in range(n + 1):
+                vid = i + (n + 1) * (j + (n + 1) * k)
+                base = np.array([i, j, k]) * h
+                if 0 < i < n and 0 < j < n and 0 < k < n:
+                    for d in range(3):
+                        st, z = _splitmix64(st)
+                        u = (z >> 11) * 2.0 ** -53
+                        assert x[vid, d] == base[d] + jit * h * (2.0 * u - 1.0)
+                else:
+                    assert np.array_equal(x[vid], base)
+    G.load_library().gdn_gdn_mesh_free(C.byref(m))
+
+
+def test_msh_writer_round_trip(tmp_path):
+    """Written coordinates read back bit for bit (%.17g)."""
+    lib = G.load_library()
+    m = G.GdnMesh()
+    lib.mesh_kuhn_cube(C.byref(m), 3, 0.3, 99)
+    x0 = np.ctypeslib.as_array(m.node, shape=(3 * m.nbnodes,)).copy()
+    e0 = np.ctypeslib.as_array(m.elem2node, shape=(4 * m.nbelems,)).copy()
+    path = str(tmp_path / "w.msh")
+    assert lib.mesh_write_msh22_file(C.byref(m), path.encode()) == 0
+    r = G.GdnMesh()
+    lib.mesh_read_from_msh22_file(C.byref(r), path.encode())
+    assert np.array_equal(np.ctypeslib.as_array(r.node, shape=(3 * r.nbnodes,)), x0)
+    assert np.array_equal(np.ctypeslib.as_array(r.elem2node, shape=(4 * r.nbelems,)), e0)
+    lib.gdn_gdn_mesh_free(C.byref(m))
+    lib.gdn_gdn_mesh_free(C.byref(r))
