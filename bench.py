@@ -67,6 +67,56 @@ def cpu_baseline(n, steps):
     return out
 
 
+RK3_BYTES_PER_ELEM_STAGE = 4 * 60 * 8 + 112  # read w, k2; write k2, w; geometry
+
+
+def bench_rk3(args):
+    """RK3, FP64, plane wave k = 2 pi (1,1,0)/sqrt 2 with E along z, Kuhn
+    64^3 x 6 with +-10% interior jitter (seed 17), CFL 0.5."""
+    n = args.n if args.n != 32 else 64
+    k = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    G.set_plane_wave(k, (0.0, 0.0, 1.0))
+    mesh = G.Mesh(kuhn=(n, 0.1, 17))
+    sim = G.Simulation(mesh, G.Model("plane_wave"))
+    sim.set_time(1.0, 0.0, 0.5)
+    nelem = mesh.nelem
+    dof = 6 * 10 * nelem
+    st = G.Stepper(sim)
+    st.advance(args.warmup)
+    st.sync()
+    t0 = time.perf_counter()
+    st.advance(args.steps)
+    st.sync()
+    elapsed = time.perf_counter() - t0
+    err = st.error_l2()
+    st.profile(True)
+    st.advance(args.profile_steps)
+    prof = st.profile(False)
+    stage_s = prof[0] / (3 * args.profile_steps)
+    achieved = RK3_BYTES_PER_ELEM_STAGE * nelem / stage_s / 1e9
+    print(f"[bench rk3] n={n} elements={nelem} dt={sim.s.dt:.4e} {args.steps} steps "
+          f"{elapsed:.3f}s stage kernel {stage_s*1e3:.3f} ms ghosts "
+          f"{prof[2]/args.profile_steps*1e3:.3f} ms/step L2 {err:.3e}",
+          file=sys.stderr, flush=True)
+    st.free()
+    sim.free()
+    line = {
+        "metric": METRIC, "value": dof * args.steps / elapsed, "unit": "DoF-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"low-storage RK3, FP64, plane wave k=2pi(1,1,0)/sqrt2 E||z, "
+                               f"Kuhn {n}^3x6 = {nelem} P2 tets, jitter 10% seed 17, CFL 0.5",
+                   "elements": nelem, "dof": dof, 
+                   "l2_error": err},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_rk3_stage", "bytes_per_element_stage": RK3_BYTES_PER_ELEM_STAGE,
+                     "avg_launch_us": stage_s * 1e6},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +126,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scheme", choices=["relax", "rk3"], default="relax",
+                    help="relax: the headline workload; rk3: low-storage RK3 "
+                         "on the jittered 64^3 plane-wave mesh (secondary)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -87,6 +140,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.scheme == "rk3":
+        return bench_rk3(args)
     n = args.n
     G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
     mesh = G.Mesh(kuhn=(n, 0.0, 0))
