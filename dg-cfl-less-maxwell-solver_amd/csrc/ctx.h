@@ -1,0 +1,163 @@
+/*
+ * Device context of the GPU layer and helpers shared by its translation
+ * units (gdh_core.hip, rk3.hip, sweep.hip).
+ */
+#ifndef GDH_CTX_H
+#define GDH_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "physics.h"
+
+#define HIPCHECK(call)                                                        \
+	do {                                                                  \
+		hipError_t err_ = (call);                                     \
+		if (err_ != hipSuccess) {                                     \
+			fprintf(stderr, "gdon3d-hip: %s failed: %s (%s:%d)\n", \
+				#call, hipGetErrorString(err_), __FILE__,     \
+				__LINE__);                                    \
+			abort();                                              \
+		}                                                             \
+	} while (0)
+
+using namespace gdh;
+
+/* ===================================================================== */
+/*                               context                                 */
+/* ===================================================================== */
+
+struct gdh_ctx {
+	int ne, nb, neq, m, scheme;
+	gdh_model_desc md;
+	ImposedPar ip;
+	KinPar kp;
+	hipStream_t stream;
+	hipEvent_t ev0, ev1;
+
+	double *geom;
+	int4 *nbr, *nbmap;
+	double *xnode, *xghost;
+	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
+	 * GH[q][6 bf + a]; kinetic: w = reference layout, ghosts included */
+	double *w[2];
+	int cur;
+	double *k2;
+	double *gh;
+	double *geom_soa; /* G[k][e], k < 10 */
+	double *stage; /* array-of-structures staging for state transfers */
+	double *hostvals; /* exact values uploaded for GDH_IMPOSED_HOST */
+	size_t hostvals_len;
+	double *partials;
+	int npartials;
+
+	/* kinetic solver */
+	double dt;
+	int nlevels;
+	std::vector<int> level_ptr;
+	int2 *items; /* (v, e) in level order */
+	int nitems;
+	double *dinv; /* per item: 10x10 inverse of the diagonal block */
+	uint8_t *cls; /* per (e, f, v): 0 none, 1 outflow, 2 inflow */
+	int *done; /* per item: epoch of the step that finished it */
+	int *status; /* nonzero: a dependency wait timed out */
+	int epoch;
+	int sweep_grid; /* resident blocks of the persistent sweep */
+	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
+
+	/* optional event timing of the main kernels: pairs of events recorded
+	 * on the stream around each kernel group, resolved when read */
+	int prof;
+	std::vector<hipEvent_t> ev_pool;
+	std::vector<int> ev_slot; /* slot of each recorded pair */
+	std::vector<hipEvent_t> ev_pairs; /* begin, end, begin, end, ... */
+	double prof_s[3];
+	long prof_launches;
+};
+
+static inline hipEvent_t ev_get(gdh_ctx *c)
+{
+	if (c->ev_pool.empty()) {
+		hipEvent_t e;
+		HIPCHECK(hipEventCreate(&e));
+		return e;
+	}
+	hipEvent_t e = c->ev_pool.back();
+	c->ev_pool.pop_back();
+	return e;
+}
+
+static inline void prof_begin(gdh_ctx *c, int slot)
+{
+	if (!c->prof)
+		return;
+	hipEvent_t e = ev_get(c);
+	HIPCHECK(hipEventRecord(e, c->stream));
+	c->ev_pairs.push_back(e);
+	c->ev_slot.push_back(slot);
+}
+
+static inline void prof_end(gdh_ctx *c, int slot)
+{
+	(void)slot;
+	if (!c->prof)
+		return;
+	hipEvent_t e = ev_get(c);
+	HIPCHECK(hipEventRecord(e, c->stream));
+	c->ev_pairs.push_back(e);
+}
+
+/* wait for the recorded pairs and accumulate their durations */
+static inline void prof_resolve(gdh_ctx *c)
+{
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	for (size_t k = 0; k < c->ev_slot.size(); k++) {
+		float ms = 0;
+		HIPCHECK(hipEventElapsedTime(&ms, c->ev_pairs[2 * k],
+					     c->ev_pairs[2 * k + 1]));
+		c->prof_s[c->ev_slot[k]] += ms * 1e-3;
+	}
+	for (hipEvent_t e : c->ev_pairs)
+		c->ev_pool.push_back(e);
+	c->ev_pairs.clear();
+	c->ev_slot.clear();
+}
+
+static inline void *dmalloc(size_t bytes)
+{
+	void *p = nullptr;
+	HIPCHECK(hipMalloc(&p, bytes ? bytes : 16));
+	return p;
+}
+
+static inline int nblocks(long n, int bs)
+{
+	return (int)((n + bs - 1) / bs);
+}
+
+/* Where the macro state of DoF ipg (component q) lives, RK3 layout. */
+struct StateRef {
+	double *w, *gh;
+	long ne, ngh;
+};
+
+__device__ inline double *macro_ptr(const StateRef &s, long ipg, int q)
+{
+	if (ipg < 10 * s.ne)
+		return s.w + (q * 10 + ipg % 10) * s.ne + ipg / 10;
+	return s.gh + q * s.ngh + (ipg - 10 * s.ne);
+}
+
+/* launchers implemented in rk3.hip and sweep.hip */
+void sweep_setup(gdh_ctx *c);
+void sweep_step(gdh_ctx *c, double *fo, double *fn);
+void relax_launch(gdh_ctx *c, double *f, long n);
+void block_inverse_launch(gdh_ctx *c);
+
+#endif

@@ -1,0 +1,239 @@
+/*
+ * Fused low-storage RK3 stage of the P2 DG Maxwell operator (FP64), one
+ * thread per element over structure-of-arrays state.
+ *
+ * HBM layout (RK3 scheme): W[q][i][e] = field q at local node i of element e
+ * (so a wave's 64 elements read 512 contiguous bytes per (q, i)), ghost
+ * values GH[q][6 bf + a], geometry G[k][e] (codtau row-major, then det).
+ *
+ * Per element, with gamma_k = column k of codtau (= det J^-T):
+ *   dw = (1/det) [ - sum_k A(gamma_k) (Minv D_k) w + sum_f (Minv L_f S) Phi_f ]
+ * where (D_k)_mj = int d_k(phi_m) phi_j (src/simulation/simulation.c:291-319,
+ * c_ij = codtau int_ref), A(n) w = (-n x H, n x E) is the Maxwell flux
+ * (src/models/maxwell.c:152-192 with wL = wR), Phi_f the upwind flux at the
+ * 6 face nodes with the scaled outward normal (src/simulation/simulation.c:
+ * 377-487), L_f S the face-mass lift and Minv / det the inverse mass
+ * (src/simulation/simulation.c:218-261). The products Minv D_k and Minv L_f S
+ * are exact tables; they are uniform across the wave and live in scalar
+ * registers. Then k2 = A k2 - dt dw, w_out = w + B k2 (src/dg/explicit.c:
+ * 82-95).
+ */
+#include "ctx.h"
+
+using namespace gdh;
+
+/* The operator tables reach the kernel through a pointer, so that they are
+ * read with scalar loads into SGPR operands instead of being materialised as
+ * FP64 immediates held in vector registers. */
+struct Rk3Tables {
+	double minv_int[3][10][10];
+	double minv_lift[4][10][6];
+};
+static const Rk3Tables h_rk3_tables = { T10_MINV_INTERNAL_INIT, T10_MINV_LIFT_INIT };
+static Rk3Tables *d_rk3_tables = nullptr;
+
+#define RK3_BS 128
+
+/* Bijection from the dispatch order of blocks to tiles such that the blocks
+ * the hardware sends to one XCD (block ids congruent mod 8) get a contiguous
+ * range of tiles: face neighbours then mostly share that XCD's L2. */
+__device__ inline int xcd_tile(int hw, int nblk)
+{
+	const int q = nblk / 8, r = nblk % 8, x = hw % 8;
+	return x * q + min(x, r) + hw / 8;
+}
+
+/* The same pointer value, hidden from the optimiser: loads through it are
+ * not merged with earlier loads of the same addresses. Re-reading the
+ * element's nodes from the L1/L2 is cheaper than keeping all of them live
+ * in registers across the kernel. */
+__device__ __forceinline__ const double *reload_ptr(const double *p)
+{
+	asm volatile("" : "+s"(p));
+	return p;
+}
+
+/* Materialise the accumulators at a phase boundary: without it the
+ * optimiser re-associates the sums across phases and sinks every product to
+ * the final update, keeping all intermediate values live. */
+__device__ __forceinline__ void pin(double (&acc)[6][10])
+{
+#pragma unroll
+	for (int p = 0; p < 6; p++)
+#pragma unroll
+		for (int m = 0; m < 10; m++)
+			asm volatile("" : "+v"(acc[p][m]));
+	__builtin_amdgcn_sched_barrier(0);
+}
+
+/* acc_p -= A(n)[p][q] h for the two outputs p that field q feeds */
+template <int Q>
+__device__ __forceinline__ void curl_accumulate(double (&acc)[6][10],
+						const double (&h)[10], double nx,
+						double ny, double nz)
+{
+	/* A(n): E-flux = -n x H, H-flux = n x E */
+	constexpr int p1 = Q == 0 ? 4 : Q == 1 ? 3 : Q == 2 ? 3 : Q == 3 ? 1 : Q == 4 ? 0 : 0;
+	constexpr int p2 = Q == 0 ? 5 : Q == 1 ? 5 : Q == 2 ? 4 : Q == 3 ? 2 : Q == 4 ? 2 : 1;
+	const double c1 = Q == 0 ? nz : Q == 1 ? -nz : Q == 2 ? ny : Q == 3 ? -nz : Q == 4 ? nz : -ny;
+	const double c2 = Q == 0 ? -ny : Q == 1 ? nx : Q == 2 ? -nx : Q == 3 ? ny : Q == 4 ? -nx : nx;
+#pragma unroll
+	for (int m = 0; m < 10; m++) {
+		acc[p1][m] -= c1 * h[m];
+		acc[p2][m] -= c2 * h[m];
+	}
+}
+
+template <int Q>
+__device__ __forceinline__ void volume_field(double (&acc)[6][10],
+					     const double *__restrict__ W, long ne,
+					     int e, const double (&ct)[9],
+					     const Rk3Tables *__restrict__ T)
+{
+	double wq[10];
+#pragma unroll
+	for (int j = 0; j < 10; j++)
+		wq[j] = W[(Q * 10 + j) * ne + e];
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		double h[10];
+#pragma unroll
+		for (int m = 0; m < 10; m++) {
+			double s = 0.0;
+#pragma unroll
+			for (int j = 0; j < 10; j++)
+				s += T->minv_int[k][m][j] * wq[j];
+			h[m] = s;
+		}
+		curl_accumulate<Q>(acc, h, ct[0 + k], ct[3 + k], ct[6 + k]);
+		pin(acc);
+	}
+}
+
+template <int F>
+__device__ __forceinline__ void face_term(double (&acc)[6][10],
+					  const double *__restrict__ W,
+					  const double *__restrict__ GH, long ne,
+					  long ngh, int e, int nb, int map,
+					  const double (&ct)[9], int bflux,
+					  const Rk3Tables *__restrict__ T)
+{
+	double vn[3];
+	face_normal(ct, F, vn);
+	const double overr =
+		1.0 / (sqrt(vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]) + 1e-16);
+#pragma unroll
+	for (int b = 0; b < 6; b++) {
+		const int i = face_node(F, b);
+		double wl[6], wr[6], fl[6];
+#pragma unroll
+		for (int q = 0; q < 6; q++)
+			wl[q] = W[(q * 10 + i) * ne + e];
+		if (nb >= 0) {
+			const int j = (map >> (4 * b)) & 15;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wr[q] = W[(q * 10 + j) * ne + nb];
+		} else {
+			const long k = 6L * (-1 - nb) + b;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wr[q] = GH[q * ngh + k];
+		}
+		if (nb < 0 && bflux == GDH_FLUX_MAXWELL_METAL)
+			maxwell_metal(wl, vn, fl);
+		else
+			maxwell_upwind(wl, wr, vn, overr, fl);
+#pragma unroll
+		for (int m = 0; m < 10; m++) {
+			const double lift = T->minv_lift[F][m][b];
+#pragma unroll
+			for (int p = 0; p < 6; p++)
+				acc[p][m] += lift * fl[p];
+		}
+		pin(acc);
+	}
+}
+
+__global__ __launch_bounds__(RK3_BS) void k_rk3_soa(
+	const double *__restrict__ W, double *__restrict__ Wo,
+	double *__restrict__ K2, const double *__restrict__ GH,
+	const double *__restrict__ G, const int4 *__restrict__ nbr,
+	const int4 *__restrict__ nbmap, const Rk3Tables *__restrict__ T, int ne_,
+	int ngh_, double Ac, double Bc, double dt, int first, int bflux)
+{
+	const long ne = ne_, ngh = ngh_;
+	const int tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int e = tile * RK3_BS + threadIdx.x;
+	if (e >= ne_)
+		return;
+	double ct[9];
+#pragma unroll
+	for (int k = 0; k < 9; k++)
+		ct[k] = G[k * ne + e];
+	const double det = G[9 * ne + e];
+
+	double acc[6][10];
+#pragma unroll
+	for (int p = 0; p < 6; p++)
+#pragma unroll
+		for (int m = 0; m < 10; m++)
+			acc[p][m] = 0.0;
+
+	volume_field<0>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+	volume_field<1>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+	volume_field<2>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+	volume_field<3>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+	volume_field<4>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+	volume_field<5>(acc, reload_ptr(W), ne, e, ct, T);
+	pin(acc);
+
+	const int4 n4 = nbr[e], m4 = nbmap[e];
+	face_term<0>(acc, reload_ptr(W), GH, ne, ngh, e, n4.x, m4.x, ct, bflux, T);
+	pin(acc);
+	face_term<1>(acc, reload_ptr(W), GH, ne, ngh, e, n4.y, m4.y, ct, bflux, T);
+	pin(acc);
+	face_term<2>(acc, reload_ptr(W), GH, ne, ngh, e, n4.z, m4.z, ct, bflux, T);
+	pin(acc);
+	face_term<3>(acc, reload_ptr(W), GH, ne, ngh, e, n4.w, m4.w, ct, bflux, T);
+	pin(acc);
+
+	const double invdet = 1.0 / det;
+	const double *Wr = reload_ptr(W);
+#pragma unroll
+	for (int q = 0; q < 6; q++)
+#pragma unroll
+		for (int i = 0; i < 10; i++) {
+			const long g = (q * 10 + i) * ne + e;
+			const double d = acc[q][i] * invdet;
+			const double k = first ? -dt * d : Ac * K2[g] - dt * d;
+			K2[g] = k;
+			Wo[g] = Wr[g] + Bc * k;
+		}
+}
+
+extern "C" void gdh_rk3_stage(gdh_ctx *c, int stage, double dt)
+{
+	static const double A[3] = { 0., -5. / 9., -153. / 128. };
+	static const double B[3] = { 1. / 3., 15. / 16., 8. / 15. };
+	const int nblk = nblocks(c->ne, RK3_BS);
+	if (!d_rk3_tables) {
+		d_rk3_tables = (Rk3Tables *)dmalloc(sizeof(Rk3Tables));
+		HIPCHECK(hipMemcpy(d_rk3_tables, &h_rk3_tables, sizeof(Rk3Tables),
+				   hipMemcpyHostToDevice));
+	}
+	prof_begin(c, 0);
+	c->prof_launches++;
+	k_rk3_soa<<<nblk, RK3_BS, 0, c->stream>>>(
+		c->w[c->cur], c->w[1 - c->cur], c->k2, c->gh, c->geom_soa, c->nbr,
+		c->nbmap, d_rk3_tables, c->ne, 6 * c->nb, A[stage], B[stage], dt, stage == 0,
+		c->md.bflux_kind);
+	HIPCHECK(hipGetLastError());
+	prof_end(c, 0);
+	c->cur = 1 - c->cur;
+}

@@ -19,6 +19,10 @@ Tables (all exact except ``FACE_SCAL``):
 * ``MASS[i][j] = int phi_i phi_j``                       (src/mesh/t10.c:57-132)
 * ``INV_MASS`` = MASS^-1                                 (src/mesh/t10.c:134-167)
 * ``INTERNAL[i][j][k] = int d_k(phi_i) phi_j``           (src/mesh/t10.c:169-691)
+* ``MINV_INTERNAL[k][m][j] = sum_l INV_MASS[m][l] INTERNAL[l][j][k]``: the
+  inverse mass applied to the volume operator of direction k;
+* ``MINV_LIFT[f][m][b] = sum_a INV_MASS[m][node(f, a)] FACE_SCAL[a][b]``:
+  the inverse mass applied to the face-f lift of the (rounded) face mass;
 * ``FACE_SCAL[a][b]``: the P2 mass matrix of the area-1/2 reference triangle
   in face-node order (v0, v1, v2, e01, e12, e20), rounded to 10 significant
   decimal digits.  The reference stores exactly that rounding
@@ -183,6 +187,23 @@ def main(out=sys.stdout):
     w("}\n\n")
     emit2("T10_FACE_SCAL_INIT", face,
           "P2 face mass, area-1/2 triangle, (v0,v1,v2,e01,e12,e20), 10 sig. digits")
+    minv_int = [[[sum(inv_mass[m][l] * internal[l][j][k] for l in range(10))
+                  for j in range(10)] for m in range(10)] for k in range(3)]
+    f2n = [(0, 3, 2, 7, 9, 6), (0, 1, 3, 4, 8, 7), (0, 2, 1, 6, 5, 4), (1, 2, 3, 5, 9, 8)]
+    minv_lift = [[[sum(inv_mass[m][f2n[f][a]] * face[a][b] for a in range(6))
+                   for b in range(6)] for m in range(10)] for f in range(4)]
+    w("/* inverse mass times the volume operator of direction k, [k][m][j] */\n"
+      "#define T10_MINV_INTERNAL_INIT { \\\n")
+    for k in range(3):
+        w("\t{ " + ", ".join("{ " + ", ".join(lit(minv_int[k][m][j]) for j in range(10)) + " }"
+                             for m in range(10)) + " }, \\\n")
+    w("}\n\n")
+    w("/* inverse mass times the lift of the rounded face mass, [f][m][b] */\n"
+      "#define T10_MINV_LIFT_INIT { \\\n")
+    for f in range(4):
+        w("\t{ " + ", ".join("{ " + ", ".join(lit(minv_lift[f][m][b]) for b in range(6)) + " }"
+                             for m in range(10)) + " }, \\\n")
+    w("}\n\n")
     w("#endif\n")
 
 

@@ -60,3 +60,23 @@ def test_relax_d3q6():
     G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
     g, o = run_pair(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(), "d3q6", 0.25, 3.0)
     assert rel_l2(g["wn"], o["wn"]) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,jitter", [(4, 0.0), (6, 0.2)])
+def test_persistent_sweep_equals_level_sweep(n, jitter, monkeypatch):
+    """The one-launch dataflow sweep (dependency flags, coherent handoff
+    between workgroups) computes bit for bit what the level-by-level launches
+    compute."""
+    path = kuhn_msh(n, jitter, 31)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    out = {}
+    for mode in ("level", "persistent"):
+        monkeypatch.setenv("GDN_SWEEP", mode)
+        mesh = G.Mesh(path=path)
+        sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic="d3q4"))
+        sim.set_time(0.6, 0.0, 5.0)
+        sim.solve()
+        out[mode] = sim.wn()
+        sim.free()
+    assert np.array_equal(out["level"], out["persistent"])
