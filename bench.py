@@ -111,7 +111,7 @@ def bench_rk3(args):
                    "l2_error": err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_rk3_stage", "bytes_per_element_stage": RK3_BYTES_PER_ELEM_STAGE,
+                     "kernel": "k_rk3_soa", "bytes_per_element_stage": RK3_BYTES_PER_ELEM_STAGE,
                      "avg_launch_us": stage_s * 1e6},
     }
     print(json.dumps(line), flush=True)
@@ -214,7 +214,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_sweep_level (all levels of one step)",
+            "kernel": "k_sweep_persistent (one launch = the whole sweep of one step)",
             "bytes_per_element_step": SWEEP_BYTES_PER_ELEM,
             "launches_per_step": launches,
             "avg_launch_us": sweep_s / max(launches, 1) * 1e6,

@@ -174,49 +174,62 @@ __device__ inline void vmem_wait()
 
 #define SWEEP_SPIN_LIMIT (1 << 20) /* polls before a wait is declared stuck */
 
-struct SweepShared {
-	double F[6][10]; /* own micro-fields of the item */
-	double U[4][6][6]; /* inflow traces ((1-theta) f + theta sol) per face */
-	double Y[6][10];
-	double R[6][10];
-	double D[100]; /* inverse diagonal block */
+/* Operator tables, copied to LDS once per block. */
+struct SweepTables {
+	double minv[10][10];
+	double S[6][6];
+};
+
+/* One prepared item: everything that does not depend on the inflow
+ * neighbours' new values. */
+struct ItemBuf {
+	double base[6][10];
+	double G[4][10][6]; /* theta G_f for the interior inflow faces */
+	int v, e;
+	int nb[4], map[4], act[4]; /* act: interior inflow face to wait on */
+};
+
+/* Work area of prepare(). */
+struct PrepScratch {
+	double F[6][10];
+	double D[100];
+	double H[10][10];
+	double U[4][6][6];
+	double Gf[4][10][6];
 };
 
 /*
- * Transport of one item (velocity v, element e), 64 lanes, lane (iw, i) =
- * (field, node):
- *   y   = (1-theta) F_ee f_e + sum_inflow s_f S ((1-theta) f_nb + theta sol_nb)
- *   rhs = f_e - dt/det Minv_ref y,   sol_e = D^-1 rhs
- * i.e. one block row of M1 sol = M2 f with M1 = I + theta dt M^-1 F_v,
- * M2 = I - (1-theta) dt M^-1 F_v (src/dg/relaxation.c:64-110), F_v restricted
- * to the element: volume -v.c_ij and outflow s_f S (src/simulation/
- * simulation.c:298-308,531-601). Ghost DoFs are identity rows of M1 and M2:
- * their sol is f. In the persistent variant the inflow neighbours' sol is
- * awaited through their done flags and read coherently.
+ * Transport of item (v, e), split around the dependency wait. With
+ * D = I + theta dt/det Minv F_ee(v) (its inverse stored per item), theta = 1/2
+ * and u_f the inflow trace ((1-theta) f_nb + theta sol_nb, or the ghost
+ * value), one block row of M1 sol = M2 f (src/dg/relaxation.c:64-110) reads
+ *   sol = D^-1 [ f - dt/det Minv ((1-theta) F_ee f + sum_f s_f L_f S u_f) ]
+ *       = 2 D^-1 f - f - sum_f G_f u_f,  G_f = s_f (dt/det) D^-1 Minv L_f S
+ * since (dt/det) Minv (1-theta) F_ee = D - I for theta = 1/2. prepare()
+ * forms 2 D^-1 f - f - sum_f G_f ((1-theta) f_nb or ghost) and theta G_f;
+ * after the wait, finish() subtracts theta G_f sol_nb. Lanes: (iw, i) =
+ * (field, node) for the vectors, (i, b) for the 10x6 matrices G_f.
  */
-template <bool PERSIST>
-__device__ inline void sweep_item(int it, const int2 *__restrict__ items,
-				  const double *__restrict__ fo, double *fn,
-				  const double *__restrict__ geom,
-				  const int4 *__restrict__ nbr,
-				  const int4 *__restrict__ nbmap,
-				  const uint8_t *__restrict__ cls,
-				  const double *__restrict__ dinv, const KinPar &kp,
-				  int ne, int neq, double dt, int *done, int epoch,
-				  int *status, SweepShared &sh)
+__device__ inline void sweep_prepare(int it, const int2 *__restrict__ items,
+				     const double *__restrict__ fo,
+				     const double *__restrict__ geom,
+				     const int4 *__restrict__ nbr,
+				     const int4 *__restrict__ nbmap,
+				     const uint8_t *__restrict__ cls,
+				     const double *__restrict__ dinv,
+				     const KinPar &kp, int ne, int neq, double dt,
+				     const SweepTables &T, ItemBuf &B, PrepScratch &P)
 {
 	const int2 ve = items[it];
 	const int v = ve.x, e = ve.y;
 	const int t = threadIdx.x;
 	const int iw = t / 10, i = t % 10;
-	const bool lane = t < 60;
 	const int nv = kp.nv;
 
-	/* independent loads first */
-	if (lane)
-		sh.F[iw][i] = fo[(long)(iw * nv + v) * neq + 10L * e + i];
+	if (t < 60)
+		P.F[iw][i] = fo[(long)(iw * nv + v) * neq + 10L * e + i];
 	for (int k = t; k < 100; k += 64)
-		sh.D[k] = dinv[100L * it + k];
+		P.D[k] = dinv[100L * it + k];
 	double ct[9];
 #pragma unroll
 	for (int k = 0; k < 9; k++)
@@ -225,15 +238,114 @@ __device__ inline void sweep_item(int it, const int2 *__restrict__ items,
 	const int4 n4 = nbr[e], m4 = nbmap[e];
 	const int nbv[4] = { n4.x, n4.y, n4.z, n4.w };
 	const int mpv[4] = { m4.x, m4.y, m4.z, m4.w };
-	int cl[4];
+	bool in[4];
 #pragma unroll
 	for (int f = 0; f < 4; f++)
-		cl[f] = cls[(4L * e + f) * nv + v];
-	const double *vv = kp.vi + 3 * v;
+		in[f] = cls[(4L * e + f) * nv + v] == 2;
+	/* inflow traces from the previous state (or the ghosts) */
+	if (t < 36) {
+		const int q = t / 6, b = t % 6;
+		const long bq = (long)(q * nv + v) * neq;
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			if (!in[f])
+				continue;
+			const int nb = nbv[f];
+			P.U[f][q][b] = nb >= 0 ?
+				(1.0 - THETA) * fo[bq + 10L * nb + ((mpv[f] >> (4 * b)) & 15)] :
+				fo[bq + 10L * ne + 6L * (-1 - nb) + b];
+		}
+	}
+	if (t == 0) {
+		B.v = v;
+		B.e = e;
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			B.nb[f] = nbv[f];
+			B.map[f] = mpv[f];
+			B.act[f] = in[f] && nbv[f] >= 0;
+		}
+	}
+	__syncthreads();
 
+	/* 2 D^-1 f - f, and H = dt/det D^-1 Minv */
+	double base = 0.0;
+	if (t < 60) {
+		double s = 0.0;
+#pragma unroll
+		for (int j = 0; j < 10; j++)
+			s += P.D[10 * i + j] * P.F[iw][j];
+		base = 2.0 * s - P.F[iw][i];
+	}
+	const double sc = dt / det;
+	for (int k = t; k < 100; k += 64) {
+		const int r = k / 10, c = k % 10;
+		double s = 0.0;
+#pragma unroll
+		for (int l = 0; l < 10; l++)
+			s += P.D[10 * r + l] * T.minv[l][c];
+		P.H[r][c] = sc * s;
+	}
+	__syncthreads();
+
+	/* G_f = s_f H[:, face f] S for each inflow face */
+	if (t < 60) {
+		const int r = t / 6, b = t % 6;
+		const double *vv = kp.vi + 3 * v;
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			if (!in[f])
+				continue;
+			const double sf = face_coef(ct, f, vv);
+			double s = 0.0;
+#pragma unroll
+			for (int a = 0; a < 6; a++)
+				s += P.H[r][face_node(f, a)] * T.S[a][b];
+			const double g = sf * s;
+			P.Gf[f][r][b] = g;
+			B.G[f][r][b] = THETA * g;
+		}
+	}
+	__syncthreads();
+
+	if (t < 60) {
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			if (!in[f])
+				continue;
+#pragma unroll
+			for (int b = 0; b < 6; b++)
+				base -= P.Gf[f][i][b] * P.U[f][iw][b];
+		}
+		B.base[iw][i] = base;
+	}
+	__syncthreads();
+}
+
+/* true when every interior inflow neighbour of the prepared item is done */
+__device__ inline bool sweep_ready(const ItemBuf &B, const int *done, int ne,
+				   int epoch)
+{
+	const int t = threadIdx.x;
+	bool wait = false;
+	if (t < 4 && B.act[t])
+		wait = __hip_atomic_load(done + (long)B.v * ne + B.nb[t],
+					 __ATOMIC_RELAXED,
+					 __HIP_MEMORY_SCOPE_AGENT) != epoch;
+	return __ballot(wait) == 0;
+}
+
+template <bool PERSIST>
+__device__ inline void sweep_finish(const ItemBuf &B, double *fn,
+				    const KinPar &kp, int ne, int neq, int *done,
+				    int epoch, int *status)
+{
+	const int t = threadIdx.x;
+	const int iw = t / 10, i = t % 10;
+	const int v = B.v, e = B.e;
 	if (PERSIST) {
-		if (t < 4 && cl[t] == 2 && nbv[t] >= 0) {
-			const int *flag = done + (long)v * ne + nbv[t];
+		if (t < 4 && B.act[t]) {
+			const int *flag = done + (long)v * ne + B.nb[t];
 			int spins = 0;
 			while (__hip_atomic_load(flag, __ATOMIC_RELAXED,
 						 __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -252,84 +364,46 @@ __device__ inline void sweep_item(int it, const int2 *__restrict__ items,
 		vmem_wait();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 	}
-
-	/* inflow traces: lane (q, b) of each inflow face */
-	if (t < 36) {
-		const int q = t / 6, b = t % 6;
-		const long bq = (long)(q * nv + v) * neq;
+	if (t < 60) {
+		const long bq = (long)(iw * kp.nv + v) * neq;
+		double s = B.base[iw][i];
 #pragma unroll
 		for (int f = 0; f < 4; f++) {
-			if (cl[f] != 2)
+			if (!B.act[f])
 				continue;
-			const int nb = nbv[f];
-			double u;
-			if (nb >= 0) {
-				const long g = bq + 10L * nb + ((mpv[f] >> (4 * b)) & 15);
-				const double sol = PERSIST ? coherent_load(fn + g) : fn[g];
-				u = (1.0 - THETA) * fo[g] + THETA * sol;
-			} else {
-				u = fo[bq + 10L * ne + 6L * (-1 - nb) + b];
-			}
-			sh.U[f][q][b] = u;
-		}
-	}
-	__syncthreads();
-
-	if (lane) {
-		double a[3];
+			const long g0 = bq + 10L * B.nb[f];
+			const int map = B.map[f];
 #pragma unroll
-		for (int k = 0; k < 3; k++)
-			a[k] = ct[0 + k] * vv[0] + ct[3 + k] * vv[1] + ct[6 + k] * vv[2];
-		double own = 0.0;
-		for (int j = 0; j < 10; j++)
-			own -= (a[0] * c_internal[i][j][0] + a[1] * c_internal[i][j][1] +
-				a[2] * c_internal[i][j][2]) *
-			       sh.F[iw][j];
-		double inflow = 0.0;
-#pragma unroll
-		for (int f = 0; f < 4; f++) {
-			const int ai = node_on_face(f, i);
-			if (ai < 0 || !cl[f])
-				continue;
-			const double s = face_coef(ct, f, vv);
-			if (cl[f] == 1) {
-				for (int b = 0; b < 6; b++)
-					own += s * c_face_scal[ai][b] * sh.F[iw][c_face2node[f][b]];
-			} else {
-				for (int b = 0; b < 6; b++)
-					inflow += s * c_face_scal[ai][b] * sh.U[f][iw][b];
+			for (int b = 0; b < 6; b++) {
+				const long g = g0 + ((map >> (4 * b)) & 15);
+				const double x = PERSIST ? coherent_load(fn + g) : fn[g];
+				s -= B.G[f][i][b] * x;
 			}
 		}
-		sh.Y[iw][i] = (1.0 - THETA) * own + inflow;
-	}
-	__syncthreads();
-	if (lane) {
-		double s = 0.0;
-		for (int m = 0; m < 10; m++)
-			s += c_inv_mass[i][m] * sh.Y[iw][m];
-		sh.R[iw][i] = sh.F[iw][i] - dt / det * s;
-	}
-	__syncthreads();
-	if (lane) {
-		double s = 0.0;
-#pragma unroll
-		for (int j = 0; j < 10; j++)
-			s += sh.D[10 * i + j] * sh.R[iw][j];
-		const long g = (long)(iw * nv + v) * neq + 10L * e + i;
+		const long g = bq + 10L * e + i;
 		if (PERSIST)
 			coherent_store(fn + g, s);
 		else
 			fn[g] = s;
 	}
 	if (PERSIST) {
-		/* the sol stores of every lane are complete before the flag */
+		/* every lane's store is complete before the flag is raised */
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 		vmem_wait();
 		if (t == 0)
-			__hip_atomic_store(done + (long)v * ne + e, epoch,
-					   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			__hip_atomic_store(done + (long)v * ne + e, epoch, __ATOMIC_RELAXED,
+					   __HIP_MEMORY_SCOPE_AGENT);
 	}
-	__syncthreads(); /* shared arrays are reused by the next item */
+	__syncthreads();
+}
+
+__device__ inline void load_tables(SweepTables &T)
+{
+	for (int k = threadIdx.x; k < 100; k += blockDim.x)
+		(&T.minv[0][0])[k] = (&c_inv_mass[0][0])[k];
+	for (int k = threadIdx.x; k < 36; k += blockDim.x)
+		(&T.S[0][0])[k] = (&c_face_scal[0][0])[k];
+	__syncthreads();
 }
 
 /* One upwind level per launch, one item per 64-lane block. */
@@ -340,9 +414,13 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	const uint8_t *__restrict__ cls, const double *__restrict__ dinv,
 	KinPar kp, int ne, int neq, double dt)
 {
-	__shared__ SweepShared sh;
-	sweep_item<false>(begin + blockIdx.x, items, fo, fn, geom, nbr, nbmap, cls,
-			  dinv, kp, ne, neq, dt, nullptr, 0, nullptr, sh);
+	__shared__ SweepTables T;
+	__shared__ ItemBuf B;
+	__shared__ PrepScratch P;
+	load_tables(T);
+	sweep_prepare(begin + blockIdx.x, items, fo, geom, nbr, nbmap, cls, dinv, kp,
+		      ne, neq, dt, T, B, P);
+	sweep_finish<false>(B, fn, kp, ne, neq, nullptr, 0, nullptr);
 }
 
 /*
@@ -351,7 +429,8 @@ __global__ __launch_bounds__(64) void k_sweep_level(
  * neighbours (always earlier items). All G blocks are resident (cooperative
  * launch), so the earliest unfinished item always has its inputs ready and
  * the sweep cannot deadlock; a bounded spin reports a timeout in *status
- * instead of hanging.
+ * instead of hanging. While an item's inputs are not ready, the block
+ * prepares its next item.
  */
 __global__ __launch_bounds__(64) void k_sweep_persistent(
 	const int2 *__restrict__ items, int nitems, const double *__restrict__ fo,
@@ -360,10 +439,27 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 	const double *__restrict__ dinv, KinPar kp, int ne, int neq, double dt,
 	int *done, int epoch, int *status)
 {
-	__shared__ SweepShared sh;
-	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
-		sweep_item<true>(it, items, fo, fn, geom, nbr, nbmap, cls, dinv, kp,
-				 ne, neq, dt, done, epoch, status, sh);
+	__shared__ SweepTables T;
+	__shared__ ItemBuf B[2];
+	__shared__ PrepScratch P;
+	load_tables(T);
+	int it = blockIdx.x, k = 0;
+	if (it < nitems)
+		sweep_prepare(it, items, fo, geom, nbr, nbmap, cls, dinv, kp, ne, neq,
+			      dt, T, B[0], P);
+	for (; it < nitems; it += gridDim.x, k ^= 1) {
+		const int next = it + gridDim.x;
+		bool prepared = false;
+		if (next < nitems && !sweep_ready(B[k], done, ne, epoch)) {
+			sweep_prepare(next, items, fo, geom, nbr, nbmap, cls, dinv, kp,
+				      ne, neq, dt, T, B[k ^ 1], P);
+			prepared = true;
+		}
+		sweep_finish<true>(B[k], fn, kp, ne, neq, done, epoch, status);
+		if (next < nitems && !prepared)
+			sweep_prepare(next, items, fo, geom, nbr, nbmap, cls, dinv, kp,
+				      ne, neq, dt, T, B[k ^ 1], P);
+	}
 }
 
 void sweep_setup(gdh_ctx *c)
@@ -415,21 +511,25 @@ void sweep_step(gdh_ctx *c, double *fo, double *fn)
 					    dim3(64), args, 0, c->stream));
 }
 
-/* f <- omega f_eq(w(f)) + (1 - omega) f at the element DoFs
- * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54) */
-__global__ void k_relax(double *__restrict__ f, KinPar kp, long n, int neq)
+/* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [0, n)
+ * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54); NV = kp.nv,
+ * fixed at compile time so that the node's 6 NV values stay in registers. */
+template <int NV>
+__global__ __launch_bounds__(256) void k_relax(double *__restrict__ f, KinPar kp,
+					       long n, int neq)
 {
 	const long ipg = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (ipg >= n)
 		return;
-	const int nv = kp.nv;
-	double fo[6 * GDH_MAX_V];
+	double fo[6][NV];
 	double w[6];
+#pragma unroll
 	for (int iw = 0; iw < 6; iw++) {
 		double s = 0.0;
-		for (int v = 0; v < nv; v++) {
-			const double x = f[(long)(iw * nv + v) * neq + ipg];
-			fo[iw * nv + v] = x;
+#pragma unroll
+		for (int v = 0; v < NV; v++) {
+			const double x = f[(long)(iw * NV + v) * neq + ipg];
+			fo[iw][v] = x;
 			s += x;
 		}
 		w[iw] = s;
@@ -437,11 +537,27 @@ __global__ void k_relax(double *__restrict__ f, KinPar kp, long n, int neq)
 	double q[6][3];
 	maxwell_qM(w, q);
 	const double om = kp.omega;
+#pragma unroll
 	for (int iw = 0; iw < 6; iw++)
-		for (int v = 0; v < nv; v++)
-			f[(long)(iw * nv + v) * neq + ipg] =
-				om * feq_comp(kp, w, q, iw, v) +
-				(1. - om) * fo[iw * nv + v];
+#pragma unroll
+		for (int v = 0; v < NV; v++)
+			f[(long)(iw * NV + v) * neq + ipg] =
+				om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v];
+}
+
+void relax_launch(gdh_ctx *c, double *f, long n)
+{
+	const int nb = nblocks(n, 256);
+	if (c->kp.nv == 4)
+		k_relax<4><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+	else if (c->kp.nv == 6)
+		k_relax<6><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+	else {
+		fprintf(stderr, "gdon3d-hip: no relaxation kernel for %d velocities\n",
+			c->kp.nv);
+		abort();
+	}
+	HIPCHECK(hipGetLastError());
 }
 
 extern "C" void gdh_relax_step(gdh_ctx *c)
@@ -455,8 +571,7 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 	prof_end(c, 0);
 	const long n = 10L * c->ne;
 	prof_begin(c, 1);
-	k_relax<<<nblocks(n, 256), 256, 0, c->stream>>>(fn, c->kp, n, c->neq);
-	HIPCHECK(hipGetLastError());
+	relax_launch(c, fn, n);
 	prof_end(c, 1);
 	c->cur = 1 - c->cur;
 }
@@ -465,8 +580,6 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 extern "C" void gdh_apply_relaxation(gdh_ctx *c)
 {
 	const long n = c->neq;
-	k_relax<<<nblocks(n, 256), 256, 0, c->stream>>>(c->w[c->cur], c->kp, n,
-							    c->neq);
-	HIPCHECK(hipGetLastError());
+	relax_launch(c, c->w[c->cur], n);
 }
 
