@@ -65,9 +65,10 @@ struct gdh_ctx {
 	int nitems;
 	double *dinv; /* per item: 10x10 inverse of the diagonal block */
 	uint8_t *cls; /* per (e, f, v): 0 none, 1 outflow, 2 inflow */
-	int *done; /* per item: epoch of the step that finished it */
+	struct ItemDesc *desc; /* per item: element, velocity, inflow faces */
+	unsigned long long *ll; /* per item: 60 values as (epoch | half) word pairs */
 	int *status; /* nonzero: a dependency wait timed out */
-	int epoch;
+	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */
 	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
 

@@ -104,7 +104,8 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 	c->items = nullptr;
 	c->dinv = nullptr;
 	c->cls = nullptr;
-	c->done = nullptr;
+	c->desc = nullptr;
+	c->ll = nullptr;
 	c->status = nullptr;
 	c->nlevels = 0;
 	c->nitems = 0;
@@ -119,7 +120,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 	void *ptrs[] = { c->geom, c->nbr,   c->nbmap,	 c->xnode,
 			 c->xghost, c->w[0], c->w[1],	 c->k2,
 			 c->hostvals, c->partials, c->items, c->dinv,
-			 c->cls,   c->done,  c->status, c->gh, c->geom_soa,
+			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa,
 			 c->stage };
 	for (void *p : ptrs)
 		if (p)

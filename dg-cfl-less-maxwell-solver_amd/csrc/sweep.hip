@@ -145,57 +145,82 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 }
 
 /* ------------------------------------------------------------------ */
-/* Cross-workgroup handoff inside one launch (persistent sweep): values   */
-/* produced by another workgroup are read and written with relaxed       */
-/* agent-scope atomics, which go through the per-XCD L2 to the coherent   */
-/* memory side, and ordered with workgroup-scope fences plus a wait for   */
-/* outstanding memory operations (no L2 flush or invalidate).             */
+/* Persistent sweep: hand-off of values between workgroups               */
 /* ------------------------------------------------------------------ */
+/*
+ * Each finished value is published as two 64-bit words (epoch << 32 | high
+ * half) and (epoch << 32 | low half), written with relaxed agent-scope
+ * stores. 64-bit accesses are single-copy atomic, so a reader that sees the
+ * current epoch in both words holds the whole value: there is no separate
+ * flag, no fence and no wait for the store to complete on the producer
+ * side, and the consumer polls the data it needs directly. The words go
+ * through the per-XCD L2 to the coherent memory side (agent scope), so the
+ * hand-off works across XCDs. The buffer starts zeroed and the epoch of the
+ * k-th sweep is k (never 0).
+ */
 typedef unsigned long long u64;
 
-__device__ inline double coherent_load(const double *p)
-{
-	const u64 v = __hip_atomic_load((const u64 *)p, __ATOMIC_RELAXED,
-					__HIP_MEMORY_SCOPE_AGENT);
-	return __longlong_as_double((long long)v);
-}
-
-__device__ inline void coherent_store(double *p, double x)
-{
-	__hip_atomic_store((u64 *)p, (u64)__double_as_longlong(x), __ATOMIC_RELAXED,
-			   __HIP_MEMORY_SCOPE_AGENT);
-}
-
-/* wait for all outstanding vector memory operations of the wave */
-__device__ inline void vmem_wait()
-{
-	__builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
-}
+struct ItemDesc {
+	double cf[4]; /* s_f dt / det on the inflow faces, 0 elsewhere */
+	int e, v;
+	int flags; /* bit f: inflow face f; bit 4 + f: inflow from the ghosts */
+	int pad;
+	int nb[4]; /* neighbour element, or the boundary face index */
+	int map[4]; /* 4 bits per face node: the neighbour's local node */
+};
 
 #define SWEEP_SPIN_LIMIT (1 << 20) /* polls before a wait is declared stuck */
 
-/* Operator tables, copied to LDS once per block. */
+__global__ void k_item_desc(const int2 *__restrict__ items, int nitems,
+			    const double *__restrict__ geom,
+			    const int4 *__restrict__ nbr,
+			    const int4 *__restrict__ nbmap,
+			    const uint8_t *__restrict__ cls, KinPar kp, double dt,
+			    ItemDesc *__restrict__ desc)
+{
+	const int it = blockIdx.x * blockDim.x + threadIdx.x;
+	if (it >= nitems)
+		return;
+	const int v = items[it].x, e = items[it].y;
+	double ct[9];
+	for (int k = 0; k < 9; k++)
+		ct[k] = geom[10L * e + k];
+	const double sc = dt / geom[10L * e + 9];
+	const int4 n4 = nbr[e], m4 = nbmap[e];
+	const int nbv[4] = { n4.x, n4.y, n4.z, n4.w };
+	const int mpv[4] = { m4.x, m4.y, m4.z, m4.w };
+	ItemDesc d;
+	d.e = e;
+	d.v = v;
+	d.flags = 0;
+	d.pad = 0;
+	for (int f = 0; f < 4; f++) {
+		const bool in = cls[(4L * e + f) * kp.nv + v] == 2;
+		d.cf[f] = in ? face_coef(ct, f, kp.vi + 3 * v) * sc : 0.0;
+		d.flags |= in ? (nbv[f] >= 0 ? 1 << f : (1 << f) | (1 << (4 + f))) : 0;
+		d.nb[f] = nbv[f] >= 0 ? nbv[f] : -1 - nbv[f];
+		d.map[f] = mpv[f];
+	}
+	desc[it] = d;
+}
+
+/* Minv L_f S: the inverse mass applied to the lift of the face mass of
+ * face f, [f][m][b] (exact rationals; the face mass rounded as the
+ * reference rounds it) */
+__constant__ static const double c_minv_lift[4][10][6] = T10_MINV_LIFT_INIT;
+
+/* Per-block operator table in LDS */
 struct SweepTables {
-	double minv[10][10];
-	double S[6][6];
+	double ML[4][10][6];
 };
 
 /* One prepared item: everything that does not depend on the inflow
- * neighbours' new values. */
+ * neighbours' new values. Slot s < nslot is the s-th interior inflow face. */
 struct ItemBuf {
 	double base[6][10];
-	double G[4][10][6]; /* theta G_f for the interior inflow faces */
-	int v, e;
-	int nb[4], map[4], act[4]; /* act: interior inflow face to wait on */
-};
-
-/* Work area of prepare(). */
-struct PrepScratch {
-	double F[6][10];
-	double D[100];
-	double H[10][10];
-	double U[4][6][6];
-	double Gf[4][10][6];
+	double G[3][10][6]; /* theta G_f of the slot's face */
+	int v, e, nslot, pad;
+	int nb[3], map[3];
 };
 
 /*
@@ -204,277 +229,366 @@ struct PrepScratch {
  * and u_f the inflow trace ((1-theta) f_nb + theta sol_nb, or the ghost
  * value), one block row of M1 sol = M2 f (src/dg/relaxation.c:64-110) reads
  *   sol = D^-1 [ f - dt/det Minv ((1-theta) F_ee f + sum_f s_f L_f S u_f) ]
- *       = 2 D^-1 f - f - sum_f G_f u_f,  G_f = s_f (dt/det) D^-1 Minv L_f S
- * since (dt/det) Minv (1-theta) F_ee = D - I for theta = 1/2. prepare()
- * forms 2 D^-1 f - f - sum_f G_f ((1-theta) f_nb or ghost) and theta G_f;
- * after the wait, finish() subtracts theta G_f sol_nb. Lanes: (iw, i) =
- * (field, node) for the vectors, (i, b) for the 10x6 matrices G_f.
+ *       = D^-1 (2 f - sum_f c_f ML_f u_f) - f,   c_f = s_f dt/det,
+ * since (dt/det) Minv (1-theta) F_ee = D - I for theta = 1/2, with
+ * ML_f = Minv L_f S. prepare() forms base = D^-1 (2 f - sum_f c_f ML_f u_f^old)
+ * - f with u^old = (1-theta) f_nb or the ghost value, and
+ * theta G_f = theta c_f D^-1 ML_f for the interior inflow faces; after the
+ * wait only sum_f theta G_f sol_nb remains. One round of global loads, then
+ * LDS only. Lanes: (iw, i) = (field, node) for vectors, (r, b) = (row, face
+ * node) for the 10x6 matrices G_f.
  */
-__device__ inline void sweep_prepare(int it, const int2 *__restrict__ items,
+__device__ inline void sweep_prepare(int it, const ItemDesc &d,
 				     const double *__restrict__ fo,
-				     const double *__restrict__ geom,
-				     const int4 *__restrict__ nbr,
-				     const int4 *__restrict__ nbmap,
-				     const uint8_t *__restrict__ cls,
-				     const double *__restrict__ dinv,
-				     const KinPar &kp, int ne, int neq, double dt,
-				     const SweepTables &T, ItemBuf &B, PrepScratch &P)
+				     const double *__restrict__ dinv, int ne,
+				     int neq, int nv, const SweepTables &T,
+				     ItemBuf &B, double (&Z)[6][10])
 {
-	const int2 ve = items[it];
-	const int v = ve.x, e = ve.y;
 	const int t = threadIdx.x;
 	const int iw = t / 10, i = t % 10;
-	const int nv = kp.nv;
+	const int r = t / 6, b6 = t % 6;
+	const int v = d.v, e = d.e;
+	const double *D = dinv + 100L * it;
 
-	if (t < 60)
-		P.F[iw][i] = fo[(long)(iw * nv + v) * neq + 10L * e + i];
-	for (int k = t; k < 100; k += 64)
-		P.D[k] = dinv[100L * it + k];
-	double ct[9];
-#pragma unroll
-	for (int k = 0; k < 9; k++)
-		ct[k] = geom[10L * e + k];
-	const double det = geom[10L * e + 9];
-	const int4 n4 = nbr[e], m4 = nbmap[e];
-	const int nbv[4] = { n4.x, n4.y, n4.z, n4.w };
-	const int mpv[4] = { m4.x, m4.y, m4.z, m4.w };
-	bool in[4];
-#pragma unroll
-	for (int f = 0; f < 4; f++)
-		in[f] = cls[(4L * e + f) * nv + v] == 2;
-	/* inflow traces from the previous state (or the ghosts) */
-	if (t < 36) {
-		const int q = t / 6, b = t % 6;
-		const long bq = (long)(q * nv + v) * neq;
+	double fv = 0.0, Dr[10], Dg[10];
+	if (t < 60) {
+		const long bq = (long)(iw * nv + v) * neq;
+		fv = fo[bq + 10L * e + i];
+		double y = 0.0;
 #pragma unroll
 		for (int f = 0; f < 4; f++) {
-			if (!in[f])
+			if (!(d.flags & (1 << f)))
 				continue;
-			const int nb = nbv[f];
-			P.U[f][q][b] = nb >= 0 ?
-				(1.0 - THETA) * fo[bq + 10L * nb + ((mpv[f] >> (4 * b)) & 15)] :
-				fo[bq + 10L * ne + 6L * (-1 - nb) + b];
+			double u[6];
+			if (d.flags & (16 << f)) {
+#pragma unroll
+				for (int b = 0; b < 6; b++)
+					u[b] = fo[bq + 10L * ne + 6L * d.nb[f] + b];
+			} else {
+#pragma unroll
+				for (int b = 0; b < 6; b++)
+					u[b] = (1.0 - THETA) *
+					       fo[bq + 10L * d.nb[f] + ((d.map[f] >> (4 * b)) & 15)];
+			}
+			double s = 0.0;
+#pragma unroll
+			for (int b = 0; b < 6; b++)
+				s += T.ML[f][i][b] * u[b];
+			y += d.cf[f] * s;
+		}
+		Z[iw][i] = 2.0 * fv - y;
+#pragma unroll
+		for (int l = 0; l < 10; l++) {
+			Dr[l] = D[10 * i + l];
+			Dg[l] = D[10 * r + l];
+		}
+		/* theta G_f = theta c_f D^-1 ML_f on the interior inflow faces */
+		int slot = 0;
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			if ((d.flags & ~(d.flags >> 4) & (1 << f)) == 0)
+				continue;
+			double s = 0.0;
+#pragma unroll
+			for (int l = 0; l < 10; l++)
+				s += Dg[l] * T.ML[f][l][b6];
+			B.G[slot][r][b6] = THETA * (d.cf[f] * s);
+			slot++;
 		}
 	}
 	if (t == 0) {
+		int slot = 0;
 		B.v = v;
 		B.e = e;
-#pragma unroll
 		for (int f = 0; f < 4; f++) {
-			B.nb[f] = nbv[f];
-			B.map[f] = mpv[f];
-			B.act[f] = in[f] && nbv[f] >= 0;
+			if ((d.flags & ~(d.flags >> 4) & (1 << f)) == 0)
+				continue;
+			B.nb[slot] = d.nb[f];
+			B.map[slot] = d.map[f];
+			slot++;
 		}
+		B.nslot = slot;
 	}
 	__syncthreads();
-
-	/* 2 D^-1 f - f, and H = dt/det D^-1 Minv */
-	double base = 0.0;
 	if (t < 60) {
-		double s = 0.0;
-#pragma unroll
-		for (int j = 0; j < 10; j++)
-			s += P.D[10 * i + j] * P.F[iw][j];
-		base = 2.0 * s - P.F[iw][i];
-	}
-	const double sc = dt / det;
-	for (int k = t; k < 100; k += 64) {
-		const int r = k / 10, c = k % 10;
 		double s = 0.0;
 #pragma unroll
 		for (int l = 0; l < 10; l++)
-			s += P.D[10 * r + l] * T.minv[l][c];
-		P.H[r][c] = sc * s;
-	}
-	__syncthreads();
-
-	/* G_f = s_f H[:, face f] S for each inflow face */
-	if (t < 60) {
-		const int r = t / 6, b = t % 6;
-		const double *vv = kp.vi + 3 * v;
-#pragma unroll
-		for (int f = 0; f < 4; f++) {
-			if (!in[f])
-				continue;
-			const double sf = face_coef(ct, f, vv);
-			double s = 0.0;
-#pragma unroll
-			for (int a = 0; a < 6; a++)
-				s += P.H[r][face_node(f, a)] * T.S[a][b];
-			const double g = sf * s;
-			P.Gf[f][r][b] = g;
-			B.G[f][r][b] = THETA * g;
-		}
-	}
-	__syncthreads();
-
-	if (t < 60) {
-#pragma unroll
-		for (int f = 0; f < 4; f++) {
-			if (!in[f])
-				continue;
-#pragma unroll
-			for (int b = 0; b < 6; b++)
-				base -= P.Gf[f][i][b] * P.U[f][iw][b];
-		}
-		B.base[iw][i] = base;
+			s += Dr[l] * Z[iw][l];
+		B.base[iw][i] = s - fv;
 	}
 	__syncthreads();
 }
 
-/* true when every interior inflow neighbour of the prepared item is done */
-__device__ inline bool sweep_ready(const ItemBuf &B, const int *done, int ne,
-				   int epoch)
+__device__ inline u64 ll_load(const u64 *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void ll_store(u64 *p, u64 x)
+{
+	__hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/*
+ * Lane s (< nslot) checks one word of slot s's neighbour trace (field 0,
+ * face node 0, high half): the full trace is read again only once every
+ * slot shows the current epoch there, so that waiting blocks do not flood
+ * the memory side with coherent loads.
+ */
+__device__ inline bool ll_sentinels(const ItemBuf &B, const u64 *ll, int ne,
+				    unsigned epoch)
 {
 	const int t = threadIdx.x;
 	bool wait = false;
-	if (t < 4 && B.act[t])
-		wait = __hip_atomic_load(done + (long)B.v * ne + B.nb[t],
-					 __ATOMIC_RELAXED,
-					 __HIP_MEMORY_SCOPE_AGENT) != epoch;
+	if (t < B.nslot) {
+		const u64 *p = ll + 2 * (((long)B.v * ne + B.nb[t]) * 60 + (B.map[t] & 15));
+		wait = (unsigned)(ll_load(p) >> 32) != epoch;
+	}
 	return __ballot(wait) == 0;
 }
 
+/*
+ * One pass over the inflow values that lane (q, b) (t < 36) still misses
+ * (bit s of pending): each complete one goes to X[s][q][b]. Returns the
+ * slots still pending.
+ */
+__device__ inline unsigned ll_poll(const ItemBuf &B, const u64 *ll, int ne,
+				   unsigned epoch, unsigned pending,
+				   double (&X)[3][6][6])
+{
+	const int t = threadIdx.x;
+	const int q = t / 6, b = t % 6;
+#pragma unroll
+	for (int s = 0; s < 3; s++) {
+		if (!(pending & (1u << s)))
+			continue;
+		const int j = (B.map[s] >> (4 * b)) & 15;
+		const u64 *p = ll + 2 * (((long)B.v * ne + B.nb[s]) * 60 + q * 10 + j);
+		const u64 hi = ll_load(p), lo = ll_load(p + 1);
+		if ((unsigned)(hi >> 32) == epoch && (unsigned)(lo >> 32) == epoch) {
+			X[s][q][b] = __longlong_as_double(
+				(long long)((hi << 32) | (lo & 0xffffffffull)));
+			pending &= ~(1u << s);
+		}
+	}
+	return pending;
+}
+
+/* bounded wait: true once the limit is passed or another wave timed out */
+__device__ inline bool spin_expired(int spins, int *status)
+{
+	if ((spins & 255) != 0)
+		return false;
+	if (spins > SWEEP_SPIN_LIMIT ||
+	    __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+		if (threadIdx.x == 0)
+			__hip_atomic_store(status, 1, __ATOMIC_RELAXED,
+					   __HIP_MEMORY_SCOPE_AGENT);
+		return true;
+	}
+	return false;
+}
+
+/* sol = base - sum_s theta G_s sol_nb; published to fn (and ll) */
 template <bool PERSIST>
-__device__ inline void sweep_finish(const ItemBuf &B, double *fn,
-				    const KinPar &kp, int ne, int neq, int *done,
-				    int epoch, int *status)
+__device__ inline void sweep_finish(const ItemBuf &B, const double (*X)[6][6],
+				    double *__restrict__ fn, u64 *ll, int nv,
+				    int ne, int neq, unsigned epoch)
 {
 	const int t = threadIdx.x;
 	const int iw = t / 10, i = t % 10;
-	const int v = B.v, e = B.e;
-	if (PERSIST) {
-		if (t < 4 && B.act[t]) {
-			const int *flag = done + (long)v * ne + B.nb[t];
-			int spins = 0;
-			while (__hip_atomic_load(flag, __ATOMIC_RELAXED,
-						 __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-				/* bounded: after a timeout anywhere every wait ends */
-				if ((++spins & 255) == 0 &&
-				    (spins > SWEEP_SPIN_LIMIT ||
-				     __hip_atomic_load(status, __ATOMIC_RELAXED,
-						       __HIP_MEMORY_SCOPE_AGENT))) {
-					__hip_atomic_store(status, 1, __ATOMIC_RELAXED,
-							   __HIP_MEMORY_SCOPE_AGENT);
-					break;
-				}
-				__builtin_amdgcn_s_sleep(1);
-			}
-		}
-		vmem_wait();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-	}
 	if (t < 60) {
-		const long bq = (long)(iw * kp.nv + v) * neq;
+		const long bq = (long)(iw * nv + B.v) * neq;
 		double s = B.base[iw][i];
 #pragma unroll
-		for (int f = 0; f < 4; f++) {
-			if (!B.act[f])
-				continue;
-			const long g0 = bq + 10L * B.nb[f];
-			const int map = B.map[f];
+		for (int sl = 0; sl < 3; sl++) {
+			if (sl >= B.nslot)
+				break;
+			if (PERSIST) {
 #pragma unroll
-			for (int b = 0; b < 6; b++) {
-				const long g = g0 + ((map >> (4 * b)) & 15);
-				const double x = PERSIST ? coherent_load(fn + g) : fn[g];
-				s -= B.G[f][i][b] * x;
+				for (int b = 0; b < 6; b++)
+					s -= B.G[sl][i][b] * X[sl][iw][b];
+			} else {
+				const long g0 = bq + 10L * B.nb[sl];
+#pragma unroll
+				for (int b = 0; b < 6; b++)
+					s -= B.G[sl][i][b] *
+					     fn[g0 + ((B.map[sl] >> (4 * b)) & 15)];
 			}
 		}
-		const long g = bq + 10L * e + i;
-		if (PERSIST)
-			coherent_store(fn + g, s);
-		else
-			fn[g] = s;
-	}
-	if (PERSIST) {
-		/* every lane's store is complete before the flag is raised */
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-		vmem_wait();
-		if (t == 0)
-			__hip_atomic_store(done + (long)v * ne + e, epoch, __ATOMIC_RELAXED,
-					   __HIP_MEMORY_SCOPE_AGENT);
+		fn[bq + 10L * B.e + i] = s;
+		if (PERSIST) {
+			const u64 bits = (u64)__double_as_longlong(s);
+			const u64 tag = (u64)epoch << 32;
+			u64 *p = ll + 2 * (((long)B.v * ne + B.e) * 60 + t);
+			ll_store(p, tag | (bits >> 32));
+			ll_store(p + 1, tag | (bits & 0xffffffffull));
+		}
 	}
 	__syncthreads();
 }
 
 __device__ inline void load_tables(SweepTables &T)
 {
-	for (int k = threadIdx.x; k < 100; k += blockDim.x)
-		(&T.minv[0][0])[k] = (&c_inv_mass[0][0])[k];
-	for (int k = threadIdx.x; k < 36; k += blockDim.x)
-		(&T.S[0][0])[k] = (&c_face_scal[0][0])[k];
+	for (int k = threadIdx.x; k < 240; k += blockDim.x)
+		(&T.ML[0][0][0])[k] = (&c_minv_lift[0][0][0])[k];
 	__syncthreads();
 }
 
 /* One upwind level per launch, one item per 64-lane block. */
 __global__ __launch_bounds__(64) void k_sweep_level(
-	const int2 *__restrict__ items, int begin, const double *__restrict__ fo,
-	double *__restrict__ fn, const double *__restrict__ geom,
-	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
-	const uint8_t *__restrict__ cls, const double *__restrict__ dinv,
-	KinPar kp, int ne, int neq, double dt)
+	const ItemDesc *__restrict__ desc, int begin, const double *__restrict__ fo,
+	double *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
+	int neq)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B;
-	__shared__ PrepScratch P;
+	__shared__ double Z[6][10];
 	load_tables(T);
-	sweep_prepare(begin + blockIdx.x, items, fo, geom, nbr, nbmap, cls, dinv, kp,
-		      ne, neq, dt, T, B, P);
-	sweep_finish<false>(B, fn, kp, ne, neq, nullptr, 0, nullptr);
+	const int it = begin + blockIdx.x;
+	const ItemDesc d = desc[it];
+	sweep_prepare(it, d, fo, dinv, ne, neq, nv, T, B, Z);
+	sweep_finish<false>(B, nullptr, fn, nullptr, nv, ne, neq, 0);
 }
 
 /*
  * The whole sweep in one launch. Items are in level order; block b takes
- * items b, b + G, b + 2G, ... and waits on the done flags of its inflow
- * neighbours (always earlier items). All G blocks are resident (cooperative
- * launch), so the earliest unfinished item always has its inputs ready and
- * the sweep cannot deadlock; a bounded spin reports a timeout in *status
- * instead of hanging. While an item's inputs are not ready, the block
- * prepares its next item.
+ * items b, b + G, b + 2G, ... and polls the values of its inflow neighbours
+ * (always earlier items). All G blocks are co-resident, so the earliest
+ * unfinished item always has its inputs ready and the sweep cannot
+ * deadlock; a bounded spin reports a timeout in *status instead of hanging.
+ * While an item's inputs are not ready, the block prepares its next item.
+ *
+ * With disc != nullptr the launch only measures residency: every block
+ * records a 100 us busy interval, and the largest number of overlapping
+ * intervals is a set of blocks that were resident together.
  */
 __global__ __launch_bounds__(64) void k_sweep_persistent(
-	const int2 *__restrict__ items, int nitems, const double *__restrict__ fo,
-	double *fn, const double *__restrict__ geom, const int4 *__restrict__ nbr,
-	const int4 *__restrict__ nbmap, const uint8_t *__restrict__ cls,
-	const double *__restrict__ dinv, KinPar kp, int ne, int neq, double dt,
-	int *done, int epoch, int *status)
+	const ItemDesc *__restrict__ desc, int nitems, const double *__restrict__ fo,
+	double *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
+	int neq, u64 *ll, unsigned epoch, int *status, u64 *disc)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
-	__shared__ PrepScratch P;
+	__shared__ double Z[6][10];
+	__shared__ double X[3][6][6];
+	const int t = threadIdx.x;
+	if (disc) {
+		if (t == 0) {
+			const u64 t0 = __builtin_amdgcn_s_memrealtime(); /* 100 MHz */
+			u64 t1 = t0;
+			for (int n = 0; n < (1 << 26) && t1 - t0 < 10000; n++)
+				t1 = __builtin_amdgcn_s_memrealtime();
+			disc[2 * blockIdx.x] = t0;
+			disc[2 * blockIdx.x + 1] = t1;
+		}
+		return;
+	}
 	load_tables(T);
 	int it = blockIdx.x, k = 0;
 	if (it < nitems)
-		sweep_prepare(it, items, fo, geom, nbr, nbmap, cls, dinv, kp, ne, neq,
-			      dt, T, B[0], P);
+		sweep_prepare(it, desc[it], fo, dinv, ne, neq, nv, T, B[0], Z);
 	for (; it < nitems; it += gridDim.x, k ^= 1) {
 		const int next = it + gridDim.x;
-		bool prepared = false;
-		if (next < nitems && !sweep_ready(B[k], done, ne, epoch)) {
-			sweep_prepare(next, items, fo, geom, nbr, nbmap, cls, dinv, kp,
-				      ne, neq, dt, T, B[k ^ 1], P);
-			prepared = true;
+		ItemDesc dn;
+		if (next < nitems)
+			dn = desc[next];
+		/* one full pass; if something is missing, prepare the next item
+		 * and then watch one word per slot */
+		unsigned pending = t < 36 ? (1u << B[k].nslot) - 1u : 0u;
+		pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+		if (__ballot(pending != 0)) {
+			bool prepared = false;
+			if (next < nitems) {
+				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+				prepared = true;
+			}
+			int spins = 0;
+			bool ready = ll_sentinels(B[k], ll, ne, epoch);
+			while (!ready) {
+				__builtin_amdgcn_s_sleep(2);
+				ready = ll_sentinels(B[k], ll, ne, epoch) ||
+					spin_expired(++spins, status);
+			}
+			pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+			while (__ballot(pending != 0)) {
+				__builtin_amdgcn_s_sleep(1);
+				pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+				if (spin_expired(++spins, status))
+					break;
+			}
+			__syncthreads(); /* X complete */
+			sweep_finish<true>(B[k], X, fn, ll, nv, ne, neq, epoch);
+			if (next < nitems && !prepared)
+				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+		} else {
+			__syncthreads(); /* X complete */
+			sweep_finish<true>(B[k], X, fn, ll, nv, ne, neq, epoch);
+			if (next < nitems)
+				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 		}
-		sweep_finish<true>(B[k], fn, kp, ne, neq, done, epoch, status);
-		if (next < nitems && !prepared)
-			sweep_prepare(next, items, fo, geom, nbr, nbmap, cls, dinv, kp,
-				      ne, neq, dt, T, B[k ^ 1], P);
 	}
 }
 
-void sweep_setup(gdh_ctx *c)
+/*
+ * Blocks of k_sweep_persistent that the device holds at once, measured: the
+ * occupancy query over-counts on gfx950 when LDS bounds the residency (it
+ * reports 20 blocks per CU for 8000 B of LDS where 18 fit), and a
+ * cooperative launch sized by it does not have all its blocks resident.
+ */
+static int sweep_residency(gdh_ctx *c)
 {
-	c->done = (int *)dmalloc(sizeof(int) * (size_t)c->nitems);
-	HIPCHECK(hipMemset(c->done, 0, sizeof(int) * (size_t)c->nitems));
-	c->status = (int *)dmalloc(sizeof(int));
-	HIPCHECK(hipMemset(c->status, 0, sizeof(int)));
-	c->epoch = 0;
 	int dev = 0, ncu = 0, per_cu = 0;
 	HIPCHECK(hipGetDevice(&dev));
 	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent,
 							      64, 0));
-	c->sweep_grid = std::max(1, per_cu * ncu);
+	const int g0 = std::max(1, per_cu * ncu);
+	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
+	k_sweep_persistent<<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
+						     nullptr, 0, 0, 0, nullptr, 0,
+						     nullptr, disc);
+	HIPCHECK(hipGetLastError());
+	std::vector<u64> h(2 * (size_t)g0);
+	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(),
+				hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	HIPCHECK(hipFree(disc));
+	/* largest overlap of the intervals [t0, t1) */
+	std::vector<std::pair<u64, int>> ev;
+	ev.reserve(2 * (size_t)g0);
+	for (int b = 0; b < g0; b++) {
+		ev.push_back({ h[2 * b], 1 });
+		ev.push_back({ h[2 * b + 1], -1 });
+	}
+	std::sort(ev.begin(), ev.end(), [](const std::pair<u64, int> &x,
+					   const std::pair<u64, int> &y) {
+		return x.first < y.first || (x.first == y.first && x.second < y.second);
+	});
+	int cur = 0, best = 0;
+	for (const auto &e : ev) {
+		cur += e.second;
+		best = std::max(best, cur);
+	}
+	return std::max(1, std::min(best, g0));
+}
+
+void sweep_setup(gdh_ctx *c)
+{
+	c->desc = (ItemDesc *)dmalloc(sizeof(ItemDesc) * (size_t)c->nitems);
+	k_item_desc<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
+		c->items, c->nitems, c->geom, c->nbr, c->nbmap, c->cls, c->kp, c->dt,
+		c->desc);
+	HIPCHECK(hipGetLastError());
+	const size_t llb = sizeof(u64) * 2 * 60 * (size_t)c->nitems;
+	c->ll = (u64 *)dmalloc(llb);
+	HIPCHECK(hipMemsetAsync(c->ll, 0, llb, c->stream));
+	c->status = (int *)dmalloc(sizeof(int));
+	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	c->epoch = 0;
+	c->sweep_grid = sweep_residency(c);
 	const char *mode = getenv("GDN_SWEEP");
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
 }
@@ -487,25 +601,27 @@ void sweep_step(gdh_ctx *c, double *fo, double *fn)
 			if (!n)
 				continue;
 			c->prof_launches++;
-			k_sweep_level<<<n, 64, 0, c->stream>>>(
-				c->items, b, fo, fn, c->geom, c->nbr, c->nbmap, c->cls,
-				c->dinv, c->kp, c->ne, c->neq, c->dt);
+			k_sweep_level<<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, c->dinv,
+							       c->kp.nv, c->ne, c->neq);
 		}
 		HIPCHECK(hipGetLastError());
 		return;
 	}
-	c->epoch++;
+	if (++c->epoch == 0) {
+		/* 2^32 sweeps: restart the epochs from a zeroed buffer */
+		HIPCHECK(hipMemsetAsync(c->ll, 0, sizeof(u64) * 2 * 60 * (size_t)c->nitems,
+					c->stream));
+		c->epoch = 1;
+	}
 	c->prof_launches++;
-	const int2 *items = c->items;
-	int nitems = c->nitems, ne = c->ne, neq = c->neq, epoch = c->epoch;
-	double dt = c->dt;
-	KinPar kp = c->kp;
-	const double *geom = c->geom, *dinv = c->dinv;
-	const int4 *nbr = c->nbr, *nbmap = c->nbmap;
-	const uint8_t *cls = c->cls;
-	int *done = c->done, *status = c->status;
-	void *args[] = { &items, &nitems, &fo, &fn, &geom, &nbr, &nbmap, &cls,
-			 &dinv, &kp, &ne, &neq, &dt, &done, &epoch, &status };
+	const ItemDesc *desc = c->desc;
+	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
+	const double *dinv = c->dinv;
+	u64 *ll = c->ll, *disc = nullptr;
+	unsigned epoch = c->epoch;
+	int *status = c->status;
+	void *args[] = { &desc, &nitems, &fo, &fn, &dinv, &nv, &ne, &neq, &ll, &epoch,
+			 &status, &disc };
 	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent,
 					    dim3(std::min(c->sweep_grid, nitems)),
 					    dim3(64), args, 0, c->stream));

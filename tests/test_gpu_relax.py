@@ -63,11 +63,12 @@ def test_relax_d3q6():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,jitter", [(4, 0.0), (6, 0.2)])
-def test_persistent_sweep_equals_level_sweep(n, jitter, monkeypatch):
-    """The one-launch dataflow sweep (dependency flags, coherent handoff
-    between workgroups) computes bit for bit what the level-by-level launches
-    compute."""
+@pytest.mark.parametrize("n,jitter,tmax", [(4, 0.0, 0.6), (6, 0.2, 0.6), (16, 0.1, 0.1)])
+def test_persistent_sweep_equals_level_sweep(n, jitter, tmax, monkeypatch):
+    """The one-launch dataflow sweep (epoch-tagged hand-off between
+    workgroups) computes bit for bit what the level-by-level launches
+    compute. At 16^3 (98,304 items) every resident block processes about 20
+    items, so the in-order item pipeline of each block is exercised."""
     path = kuhn_msh(n, jitter, 31)
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
     out = {}
@@ -75,7 +76,7 @@ def test_persistent_sweep_equals_level_sweep(n, jitter, monkeypatch):
         monkeypatch.setenv("GDN_SWEEP", mode)
         mesh = G.Mesh(path=path)
         sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic="d3q4"))
-        sim.set_time(0.6, 0.0, 5.0)
+        sim.set_time(tmax, 0.0, 5.0)
         sim.solve()
         out[mode] = sim.wn()
         sim.free()
