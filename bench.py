@@ -35,6 +35,13 @@ import gdon3d as G  # noqa: E402
 METRIC = "DoF-updates/s at fixed L2 error, P2 Maxwell RK3 & relaxation; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
+# HBM bytes actually moved per launch, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes calibrated against a 1 GiB copy (FETCH_SIZE reads half the
+# bytes, WRITE_SIZE all of them); per element of the workloads measured there
+# (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
+# D3Q4 and one RK3 stage at 64^3 jittered.
+SWEEP_TRAFFIC_BYTES_PER_ELEM = 6.245e9 / 196608
+RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (2 * 2634964.1 + 1474560.0) * 1024 / 1572864
 
 
 def cpu_baseline(n, steps):
@@ -110,11 +117,24 @@ def bench_rk3(args):
                    "elements": nelem, "dof": dof, 
                    "l2_error": err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem,
+                     "traffic_gb_s": RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem / stage_s / 1e9,
                      "kernel": "k_rk3_soa", "bytes_per_element_stage": RK3_BYTES_PER_ELEM_STAGE,
                      "avg_launch_us": stage_s * 1e6},
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration(args):
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch, kernel k_copy_calib), run under the same rocprofv3 --pmc passes
+    as the solver kernels: FETCH_SIZE / WRITE_SIZE of this copy over 1 GiB is
+    the factor by which the counters over- or under-count HBM bytes."""
+    lib = G.load_library()
+    ms = lib.gdh_copy_calibration(1 << 30, args.steps)
+    print(f"[copy] 1 GiB copy: {ms:.3f} ms, {2 * (1 << 30) / ms / 1e6:.1f} GB/s",
+          file=sys.stderr, flush=True)
 
 
 def main():
@@ -126,9 +146,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scheme", choices=["relax", "rk3"], default="relax",
+    ap.add_argument("--scheme", choices=["relax", "rk3", "copy"], default="relax",
                     help="relax: the headline workload; rk3: low-storage RK3 "
-                         "on the jittered 64^3 plane-wave mesh (secondary)")
+                         "on the jittered 64^3 plane-wave mesh (secondary); copy: "
+                         "1 GiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -142,6 +163,8 @@ def main():
 
     if args.scheme == "rk3":
         return bench_rk3(args)
+    if args.scheme == "copy":
+        return copy_calibration(args)
     n = args.n
     G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
     mesh = G.Mesh(kuhn=(n, 0.0, 0))
@@ -213,7 +236,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem,
+            "traffic_gb_s": SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem / sweep_s / 1e9,
             "kernel": "k_sweep_persistent (one launch = the whole sweep of one step)",
             "bytes_per_element_step": SWEEP_BYTES_PER_ELEM,
             "launches_per_step": launches,

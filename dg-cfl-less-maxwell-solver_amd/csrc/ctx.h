@@ -50,7 +50,8 @@ struct gdh_ctx {
 	int cur;
 	double *k2;
 	double *gh;
-	double *geom_soa; /* G[k][e], k < 10 */
+	double *geom_soa; /* G[k][c], k < 10, by storage column c */
+	int *elem_pos; /* RK3: storage column of each mesh element */
 	double *stage; /* array-of-structures staging for state transfers */
 	double *hostvals; /* exact values uploaded for GDH_IMPOSED_HOST */
 	size_t hostvals_len;
@@ -143,15 +144,18 @@ static inline int nblocks(long n, int bs)
 }
 
 /* Where the macro state of DoF ipg (component q) lives, RK3 layout. */
+/* RK3 state: element DoFs in SoA, element e of the mesh stored at column
+ * pos[e] (a locality order, see elem_order in gdh_core.hip), ghosts apart */
 struct StateRef {
 	double *w, *gh;
+	const int *pos;
 	long ne, ngh;
 };
 
 __device__ inline double *macro_ptr(const StateRef &s, long ipg, int q)
 {
 	if (ipg < 10 * s.ne)
-		return s.w + (q * 10 + ipg % 10) * s.ne + ipg / 10;
+		return s.w + (q * 10 + ipg % 10) * s.ne + s.pos[ipg / 10];
 	return s.gh + q * s.ngh + (ipg - 10 * s.ne);
 }
 

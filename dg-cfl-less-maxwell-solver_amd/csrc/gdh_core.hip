@@ -20,6 +20,56 @@ extern "C" int gdh_device_count(void)
 	return n;
 }
 
+/*
+ * Storage order of the RK3 state: elements sorted along a Morton (Z-order)
+ * curve of their vertex centroids (21 bits per axis over the bounding box,
+ * ties by mesh index). Face neighbours then sit close together in memory,
+ * and the blocks that one XCD runs (a contiguous tile range) cover a compact
+ * region whose neighbour traces stay in that XCD's L2; in mesh order the
+ * neighbours across z-layers are 6 n^2 elements apart and their traces are
+ * re-fetched from HBM. Returns order[c] = mesh element stored at column c.
+ */
+static std::vector<int> elem_order(const gdh_mesh_desc *mesh)
+{
+	const long ne = mesh->nelem;
+	std::vector<double> cen(3 * ne);
+	double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+	for (long e = 0; e < ne; e++)
+		for (int d = 0; d < 3; d++) {
+			double s = 0.0;
+			for (int i = 0; i < 4; i++)
+				s += mesh->xnode[30 * e + 3 * i + d];
+			cen[3 * e + d] = 0.25 * s;
+			lo[d] = std::min(lo[d], cen[3 * e + d]);
+			hi[d] = std::max(hi[d], cen[3 * e + d]);
+		}
+	auto spread = [](uint64_t x) { /* 21 bits -> every third bit */
+		x &= 0x1fffff;
+		x = (x | x << 32) & 0x1f00000000ffffull;
+		x = (x | x << 16) & 0x1f0000ff0000ffull;
+		x = (x | x << 8) & 0x100f00f00f00f00full;
+		x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+		x = (x | x << 2) & 0x1249249249249249ull;
+		return x;
+	};
+	std::vector<std::pair<uint64_t, int>> key(ne);
+	for (long e = 0; e < ne; e++) {
+		uint64_t code = 0;
+		for (int d = 0; d < 3; d++) {
+			const double ext = hi[d] - lo[d];
+			const double u = ext > 0 ? (cen[3 * e + d] - lo[d]) / ext : 0.0;
+			const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, u * 2097151.0));
+			code |= spread(q) << d;
+		}
+		key[e] = { code, (int)e };
+	}
+	std::sort(key.begin(), key.end());
+	std::vector<int> order(ne);
+	for (long c = 0; c < ne; c++)
+		order[c] = key[c].second;
+	return order;
+}
+
 extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 			       const gdh_model_desc *model, int scheme)
 {
@@ -61,10 +111,6 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 	c->xghost = (double *)dmalloc(sizeof(double) * 18 * (nb ? nb : 1));
 	HIPCHECK(hipMemcpy(c->geom, mesh->geom, sizeof(double) * 10 * ne,
 			   hipMemcpyHostToDevice));
-	HIPCHECK(hipMemcpy(c->nbr, mesh->nbr, sizeof(int4) * ne,
-			   hipMemcpyHostToDevice));
-	HIPCHECK(hipMemcpy(c->nbmap, mesh->nbmap, sizeof(int4) * ne,
-			   hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(c->xnode, mesh->xnode, sizeof(double) * 30 * ne,
 			   hipMemcpyHostToDevice));
 	if (nb)
@@ -82,16 +128,47 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 	c->k2 = nullptr;
 	c->gh = nullptr;
 	c->geom_soa = nullptr;
+	c->elem_pos = nullptr;
 	c->stage = nullptr;
-	if (!scheme) {
+	if (scheme) {
+		/* the sweep reads neighbours in mesh order */
+		HIPCHECK(hipMemcpy(c->nbr, mesh->nbr, sizeof(int4) * ne,
+				   hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(c->nbmap, mesh->nbmap, sizeof(int4) * ne,
+				   hipMemcpyHostToDevice));
+	} else {
+		/* RK3 works on storage columns: geometry, neighbours (as columns)
+		 * and face maps permuted, ghosts unchanged */
+		const std::vector<int> order = elem_order(mesh);
+		std::vector<int> pos(ne);
+		for (size_t col = 0; col < ne; col++)
+			pos[order[col]] = (int)col;
+		std::vector<double> gs(10 * ne);
+		std::vector<int4> nb4(ne), nm4(ne);
+		for (size_t col = 0; col < ne; col++) {
+			const size_t e = order[col];
+			for (int k = 0; k < 10; k++)
+				gs[k * ne + col] = mesh->geom[10 * e + k];
+			int nn[4];
+			for (int f = 0; f < 4; f++) {
+				const int x = mesh->nbr[4 * e + f];
+				nn[f] = x >= 0 ? pos[x] : x;
+			}
+			nb4[col] = make_int4(nn[0], nn[1], nn[2], nn[3]);
+			nm4[col] = make_int4(mesh->nbmap[4 * e], mesh->nbmap[4 * e + 1],
+					     mesh->nbmap[4 * e + 2], mesh->nbmap[4 * e + 3]);
+		}
+		HIPCHECK(hipMemcpy(c->nbr, nb4.data(), sizeof(int4) * ne,
+				   hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(c->nbmap, nm4.data(), sizeof(int4) * ne,
+				   hipMemcpyHostToDevice));
+		c->elem_pos = (int *)dmalloc(sizeof(int) * ne);
+		HIPCHECK(hipMemcpy(c->elem_pos, pos.data(), sizeof(int) * ne,
+				   hipMemcpyHostToDevice));
 		c->k2 = (double *)dmalloc(sizeof(double) * blen);
 		HIPCHECK(hipMemset(c->k2, 0, sizeof(double) * blen));
 		c->gh = (double *)dmalloc(sizeof(double) * 36 * (nb ? nb : 1));
 		HIPCHECK(hipMemset(c->gh, 0, sizeof(double) * 36 * (nb ? nb : 1)));
-		std::vector<double> gs(10 * ne);
-		for (size_t e = 0; e < ne; e++)
-			for (int k = 0; k < 10; k++)
-				gs[k * ne + e] = mesh->geom[10 * e + k];
 		c->geom_soa = (double *)dmalloc(sizeof(double) * 10 * ne);
 		HIPCHECK(hipMemcpy(c->geom_soa, gs.data(), sizeof(double) * 10 * ne,
 				   hipMemcpyHostToDevice));
@@ -120,7 +197,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 	void *ptrs[] = { c->geom, c->nbr,   c->nbmap,	 c->xnode,
 			 c->xghost, c->w[0], c->w[1],	 c->k2,
 			 c->hostvals, c->partials, c->items, c->dinv,
-			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa,
+			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa, c->elem_pos,
 			 c->stage };
 	for (void *p : ptrs)
 		if (p)
@@ -158,6 +235,7 @@ static StateRef state_ref(gdh_ctx *c)
 	StateRef s;
 	s.w = c->w[c->cur];
 	s.gh = c->gh;
+	s.pos = c->elem_pos;
 	s.ne = c->ne;
 	s.ngh = 6L * c->nb;
 	return s;

@@ -136,6 +136,7 @@ def load_library(path=LIB_PATH):
         "maxwell_num_flux_upwind": (None, [c_double_p, c_double_p, c_double_p, c_double_p]),
         "maxwell_num_flux_boundary_metal": (None, [c_double_p, c_double_p, c_double_p, c_double_p]),
         "gdh_device_count": (C.c_int, []),
+        "gdh_copy_calibration": (C.c_double, [C.c_size_t, C.c_int]),
         "gdn_upwind_classes": (C.POINTER(C.c_uint8), [P(GdnMesh), P(GdnModel)]),
         "dg_stepper_create": (C.c_void_p, [P(GdnSimulation)]),
         "dg_stepper_init": (None, [C.c_void_p]),

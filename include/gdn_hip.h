@@ -142,6 +142,13 @@ void gdh_synchronize(gdh_ctx *ctx);
 void gdh_timer_start(gdh_ctx *ctx);
 double gdh_timer_stop(gdh_ctx *ctx);
 
+/* Copies `bytes` (read) into a second buffer (written) `reps` times with a
+ * plain streaming kernel (k_copy_calib) after one warm-up launch; returns
+ * the mean milliseconds per launch. No counterpart in the reference: the
+ * benchmark uses it for a measured copy bandwidth and to calibrate the
+ * profiler's HBM byte counters against a kernel of known traffic. */
+double gdh_copy_calibration(size_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif
