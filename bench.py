@@ -78,8 +78,9 @@ RK3_BYTES_PER_ELEM_STAGE = 4 * 60 * 8 + 112  # read w, k2; write k2, w; geometry
 
 
 def bench_rk3(args):
-    """RK3, FP64, plane wave k = 2 pi (1,1,0)/sqrt 2 with E along z, Kuhn
-    64^3 x 6 with +-10% interior jitter (seed 17), CFL 0.5."""
+    """RK3, plane wave k = 2 pi (1,1,0)/sqrt 2 with E along z, Kuhn 64^3 x 6
+    with +-10% interior jitter (seed 17), CFL 0.5; --precision fp32 is the
+    precision form (binary32 fields and flux/volume arithmetic)."""
     n = args.n if args.n != 32 else 64
     k = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
     G.set_plane_wave(k, (0.0, 0.0, 1.0))
@@ -88,7 +89,7 @@ def bench_rk3(args):
     sim.set_time(1.0, 0.0, 0.5)
     nelem = mesh.nelem
     dof = 6 * 10 * nelem
-    st = G.Stepper(sim)
+    st = G.Stepper(sim, precision=args.precision)
     st.advance(args.warmup)
     st.sync()
     t0 = time.perf_counter()
@@ -100,7 +101,9 @@ def bench_rk3(args):
     st.advance(args.profile_steps)
     prof = st.profile(False)
     stage_s = prof[0] / (3 * args.profile_steps)
-    achieved = RK3_BYTES_PER_ELEM_STAGE * nelem / stage_s / 1e9
+    s_val = 4 if args.precision == "fp32" else 8
+    bytes_elem = 4 * 60 * s_val + 112
+    achieved = bytes_elem * nelem / stage_s / 1e9
     print(f"[bench rk3] n={n} elements={nelem} dt={sim.s.dt:.4e} {args.steps} steps "
           f"{elapsed:.3f}s stage kernel {stage_s*1e3:.3f} ms ghosts "
           f"{prof[2]/args.profile_steps*1e3:.3f} ms/step L2 {err:.3e}",
@@ -111,16 +114,17 @@ def bench_rk3(args):
         "metric": METRIC, "value": dof * args.steps / elapsed, "unit": "DoF-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"low-storage RK3, FP64, plane wave k=2pi(1,1,0)/sqrt2 E||z, "
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"low-storage RK3, {args.precision.upper()}, plane wave k=2pi(1,1,0)/sqrt2 E||z, "
                                f"Kuhn {n}^3x6 = {nelem} P2 tets, jitter 10% seed 17, CFL 0.5",
                    "elements": nelem, "dof": dof, 
                    "l2_error": err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem,
-                     "traffic_gb_s": RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem / stage_s / 1e9,
-                     "kernel": "k_rk3_soa", "bytes_per_element_stage": RK3_BYTES_PER_ELEM_STAGE,
+                     "traffic": (RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem
+                                 if args.precision == "fp64" and n == 64 else None),
+                     "kernel": "k_rk3_soa", "bytes_per_element_stage": bytes_elem,
                      "avg_launch_us": stage_s * 1e6},
     }
     print(json.dumps(line), flush=True)
@@ -146,6 +150,13 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fp32: the precision form (binary32 fields)")
+    ap.add_argument("--jitter", type=float, default=0.0,
+                    help="relax: interior vertex jitter (fraction of h, seed 17)")
+    ap.add_argument("--exact", choices=["cavity", "plane"], default="cavity",
+                    help="relax: PEC cavity mode (1,1,1) to t=2, or the oblique "
+                         "plane wave k=2pi(1,1,0)/sqrt2 E||z to t=1")
     ap.add_argument("--scheme", choices=["relax", "rk3", "copy"], default="relax",
                     help="relax: the headline workload; rk3: low-storage RK3 "
                          "on the jittered 64^3 plane-wave mesh (secondary); copy: "
@@ -166,18 +177,31 @@ def main():
     if args.scheme == "copy":
         return copy_calibration(args)
     n = args.n
-    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
-    mesh = G.Mesh(kuhn=(n, 0.0, 0))
-    model = G.Model("cavity_mode", kinetic="d3q4")
+    if args.exact == "cavity":
+        G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+        mesh = G.Mesh(kuhn=(n, args.jitter, 17))
+        model = G.Model("cavity_mode", kinetic="d3q4")
+        tend = 2.0
+        what = "PEC cube mode (1,1,1)"
+    else:
+        kv = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+        G.set_plane_wave(kv, (0.0, 0.0, 1.0))
+        mesh = G.Mesh(kuhn=(n, args.jitter, 17))
+        model = G.Model("plane_wave", kinetic="d3q4")
+        tend = 1.0
+        what = "plane wave k=2pi(1,1,0)/sqrt2 E||z"
     sim = G.Simulation(mesh, model)
-    sim.set_time(2.0, 0.0, 5.0)
+    sim.set_time(tend, 0.0, 5.0)
     nelem = mesh.nelem
     dof = 24 * 10 * nelem
 
     t0 = time.perf_counter()
-    st = G.Stepper(sim)
+    st = G.Stepper(sim, precision=args.precision)
     st.sync()
     t_setup = time.perf_counter() - t0
+    sim_itermax = sim.s.itermax
+    default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
+                        and args.precision == "fp64")
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
           f"levels={st.levels} setup={t_setup:.2f}s", file=sys.stderr, flush=True)
 
@@ -199,7 +223,9 @@ def main():
     sweep_s = prof[0] / args.profile_steps
     relax_s = prof[1] / args.profile_steps
     launches = prof[3] / args.profile_steps
-    achieved = SWEEP_BYTES_PER_ELEM * nelem / sweep_s / 1e9
+    s_val = 4 if args.precision == "fp32" else 8
+    sweep_bytes = 480 * s_val + 112 + 16
+    achieved = sweep_bytes * nelem / sweep_s / 1e9
     print(f"[bench] per step: sweep {sweep_s*1e3:.3f} ms ({launches:.0f} launches), "
           f"relax {relax_s*1e3:.3f} ms, ghosts {prof[2]/args.profile_steps*1e3:.3f} ms",
           file=sys.stderr, flush=True)
@@ -218,14 +244,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"CFL-less relaxation, D3Q4, FP64, PEC cube mode (1,1,1), "
-                        f"Kuhn {n}^3x6 = {nelem} P2 tets, cfl 5, t_end 2",
+            "workload": f"CFL-less relaxation, D3Q4, {args.precision.upper()}, {what}, "
+                        f"Kuhn {n}^3x6 = {nelem} P2 tets"
+                        f"{f', jitter {args.jitter:g} seed 17' if args.jitter else ''}, "
+                        f"cfl 5, t_end {tend:g}",
             "elements": nelem,
             "dof": dof,
-            "itermax": 321 if n == 32 else None,
+            "itermax": sim_itermax,
             "t_final": t_final,
             "l2_error": err,
             "hbm_gb_s_sweep": achieved,
@@ -236,15 +264,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem,
-            "traffic_gb_s": SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem / sweep_s / 1e9,
+            "traffic": (SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem if default_workload else None),
+            "traffic_gb_s": (SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem / sweep_s / 1e9
+                             if default_workload else None),
             "kernel": "k_sweep_persistent (one launch = the whole sweep of one step)",
-            "bytes_per_element_step": SWEEP_BYTES_PER_ELEM,
+            "bytes_per_element_step": sweep_bytes,
             "launches_per_step": launches,
             "avg_launch_us": sweep_s / max(launches, 1) * 1e6,
         },
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and default_workload:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(line), flush=True)
 

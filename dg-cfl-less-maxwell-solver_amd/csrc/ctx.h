@@ -35,6 +35,8 @@ using namespace gdh;
 
 struct gdh_ctx {
 	int ne, nb, neq, m, scheme;
+	int fp32; /* state stored (and fluxes computed) in binary32 */
+	size_t rsz; /* bytes per state value: 8, or 4 with fp32 */
 	gdh_model_desc md;
 	ImposedPar ip;
 	KinPar kp;
@@ -45,11 +47,12 @@ struct gdh_ctx {
 	int4 *nbr, *nbmap;
 	double *xnode, *xghost;
 	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
-	 * GH[q][6 bf + a]; kinetic: w = reference layout, ghosts included */
-	double *w[2];
+	 * GH[q][6 bf + a]; kinetic: w = reference layout, ghosts included.
+	 * Elements are double, or float with fp32. */
+	void *w[2];
 	int cur;
-	double *k2;
-	double *gh;
+	void *k2;
+	void *gh;
 	double *geom_soa; /* G[k][c], k < 10, by storage column c */
 	int *elem_pos; /* RK3: storage column of each mesh element */
 	double *stage; /* array-of-structures staging for state transfers */
@@ -67,7 +70,7 @@ struct gdh_ctx {
 	double *dinv; /* per item: 10x10 inverse of the diagonal block */
 	uint8_t *cls; /* per (e, f, v): 0 none, 1 outflow, 2 inflow */
 	struct ItemDesc *desc; /* per item: element, velocity, inflow faces */
-	unsigned long long *ll; /* per item: 60 values as (epoch | half) word pairs */
+	unsigned long long *ll; /* per item: 60 values as epoch-tagged words */
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */
@@ -145,14 +148,16 @@ static inline int nblocks(long n, int bs)
 
 /* Where the macro state of DoF ipg (component q) lives, RK3 layout. */
 /* RK3 state: element DoFs in SoA, element e of the mesh stored at column
- * pos[e] (a locality order, see elem_order in gdh_core.hip), ghosts apart */
-struct StateRef {
-	double *w, *gh;
+ * pos[e] (a locality order, see elem_order in gdh_core.hip), ghosts apart;
+ * T = double or float */
+template <typename T> struct StateRef {
+	T *w, *gh;
 	const int *pos;
 	long ne, ngh;
 };
 
-__device__ inline double *macro_ptr(const StateRef &s, long ipg, int q)
+template <typename T>
+__device__ inline T *macro_ptr(const StateRef<T> &s, long ipg, int q)
 {
 	if (ipg < 10 * s.ne)
 		return s.w + (q * 10 + ipg % 10) * s.ne + s.pos[ipg / 10];
@@ -161,8 +166,8 @@ __device__ inline double *macro_ptr(const StateRef &s, long ipg, int q)
 
 /* launchers implemented in rk3.hip and sweep.hip */
 void sweep_setup(gdh_ctx *c);
-void sweep_step(gdh_ctx *c, double *fo, double *fn);
-void relax_launch(gdh_ctx *c, double *f, long n);
+void sweep_step(gdh_ctx *c, void *fo, void *fn);
+void relax_launch(gdh_ctx *c, void *f, long n);
 void block_inverse_launch(gdh_ctx *c);
 
 #endif

@@ -73,6 +73,13 @@ static std::vector<int> elem_order(const gdh_mesh_desc *mesh)
 extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 			       const gdh_model_desc *model, int scheme)
 {
+	return gdh_create_ex(mesh, model, scheme, GDH_FP64);
+}
+
+extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
+				  const gdh_model_desc *model, int scheme,
+				  int precision)
+{
 	if (gdh_device_count() < 1) {
 		fprintf(stderr, "gdon3d-hip: no HIP device available\n");
 		abort();
@@ -82,6 +89,12 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 	c->nb = mesh->nbface;
 	c->neq = 10 * c->ne + 6 * c->nb;
 	c->scheme = scheme;
+	if (precision != GDH_FP64 && precision != GDH_FP32) {
+		fprintf(stderr, "gdon3d-hip: unknown precision %d\n", precision);
+		abort();
+	}
+	c->fp32 = precision == GDH_FP32;
+	c->rsz = c->fp32 ? sizeof(float) : sizeof(double);
 	c->md = *model;
 	c->m = scheme ? model->nb_v * 6 : 6;
 	c->ip.kind = model->imposed_kind;
@@ -121,8 +134,8 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 	/* RK3: interior SoA buffers; kinetic: full reference-layout buffers */
 	const size_t blen = scheme ? wlen : 60 * ne;
 	for (int b = 0; b < 2; b++) {
-		c->w[b] = (double *)dmalloc(sizeof(double) * blen);
-		HIPCHECK(hipMemset(c->w[b], 0, sizeof(double) * blen));
+		c->w[b] = dmalloc(c->rsz * blen);
+		HIPCHECK(hipMemset(c->w[b], 0, c->rsz * blen));
 	}
 	c->cur = 0;
 	c->k2 = nullptr;
@@ -165,15 +178,17 @@ extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 		c->elem_pos = (int *)dmalloc(sizeof(int) * ne);
 		HIPCHECK(hipMemcpy(c->elem_pos, pos.data(), sizeof(int) * ne,
 				   hipMemcpyHostToDevice));
-		c->k2 = (double *)dmalloc(sizeof(double) * blen);
-		HIPCHECK(hipMemset(c->k2, 0, sizeof(double) * blen));
-		c->gh = (double *)dmalloc(sizeof(double) * 36 * (nb ? nb : 1));
-		HIPCHECK(hipMemset(c->gh, 0, sizeof(double) * 36 * (nb ? nb : 1)));
+		c->k2 = dmalloc(c->rsz * blen);
+		HIPCHECK(hipMemset(c->k2, 0, c->rsz * blen));
+		c->gh = dmalloc(c->rsz * 36 * (nb ? nb : 1));
+		HIPCHECK(hipMemset(c->gh, 0, c->rsz * 36 * (nb ? nb : 1)));
 		c->geom_soa = (double *)dmalloc(sizeof(double) * 10 * ne);
 		HIPCHECK(hipMemcpy(c->geom_soa, gs.data(), sizeof(double) * 10 * ne,
 				   hipMemcpyHostToDevice));
 		c->stage = (double *)dmalloc(sizeof(double) * wlen);
 	}
+	if (scheme && c->fp32)
+		c->stage = (double *)dmalloc(sizeof(double) * wlen);
 	c->hostvals = nullptr;
 	c->hostvals_len = 0;
 	c->npartials = 1024;
@@ -213,28 +228,45 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 
 /* RK3 layout conversions between the reference AoS wn[ipg * 6 + q] and
  * the device SoA state */
-__global__ void k_aos_to_soa(const double *__restrict__ a, StateRef s, long n)
+template <typename T>
+__global__ void k_aos_to_soa(const double *__restrict__ a, StateRef<T> s, long n)
 {
 	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
 		return;
 	const long ipg = k / 6;
-	*macro_ptr(s, ipg, (int)(k % 6)) = a[k];
+	*macro_ptr(s, ipg, (int)(k % 6)) = (T)a[k];
 }
 
-__global__ void k_soa_to_aos(double *__restrict__ a, StateRef s, long n)
+template <typename T>
+__global__ void k_soa_to_aos(double *__restrict__ a, StateRef<T> s, long n)
 {
 	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
 		return;
-	a[k] = *macro_ptr(s, k / 6, (int)(k % 6));
+	a[k] = (double)*macro_ptr(s, k / 6, (int)(k % 6));
 }
 
-static StateRef state_ref(gdh_ctx *c)
+/* kinetic state in the reference layout, binary64 <-> binary32 */
+__global__ void k_narrow(const double *__restrict__ a, float *__restrict__ b, long n)
 {
-	StateRef s;
-	s.w = c->w[c->cur];
-	s.gh = c->gh;
+	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n)
+		b[k] = (float)a[k];
+}
+
+__global__ void k_widen(const float *__restrict__ b, double *__restrict__ a, long n)
+{
+	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n)
+		a[k] = (double)b[k];
+}
+
+template <typename T> static StateRef<T> state_ref(gdh_ctx *c)
+{
+	StateRef<T> s;
+	s.w = (T *)c->w[c->cur];
+	s.gh = (T *)c->gh;
 	s.pos = c->elem_pos;
 	s.ne = c->ne;
 	s.ngh = 6L * c->nb;
@@ -244,14 +276,21 @@ static StateRef state_ref(gdh_ctx *c)
 extern "C" void gdh_upload_state(gdh_ctx *c, const double *wn)
 {
 	const size_t len = (size_t)c->m * c->neq;
-	if (c->scheme) {
+	if (c->scheme && !c->fp32) {
 		HIPCHECK(hipMemcpyAsync(c->w[c->cur], wn, sizeof(double) * len,
 					hipMemcpyHostToDevice, c->stream));
 	} else {
 		HIPCHECK(hipMemcpyAsync(c->stage, wn, sizeof(double) * len,
 					hipMemcpyHostToDevice, c->stream));
-		k_aos_to_soa<<<nblocks(len, 256), 256, 0, c->stream>>>(
-			c->stage, state_ref(c), (long)len);
+		if (c->scheme)
+			k_narrow<<<nblocks(len, 256), 256, 0, c->stream>>>(
+				c->stage, (float *)c->w[c->cur], (long)len);
+		else if (c->fp32)
+			k_aos_to_soa<float><<<nblocks(len, 256), 256, 0, c->stream>>>(
+				c->stage, state_ref<float>(c), (long)len);
+		else
+			k_aos_to_soa<double><<<nblocks(len, 256), 256, 0, c->stream>>>(
+				c->stage, state_ref<double>(c), (long)len);
 		HIPCHECK(hipGetLastError());
 	}
 	HIPCHECK(hipStreamSynchronize(c->stream));
@@ -260,12 +299,19 @@ extern "C" void gdh_upload_state(gdh_ctx *c, const double *wn)
 extern "C" void gdh_download_state(gdh_ctx *c, double *wn)
 {
 	const size_t len = (size_t)c->m * c->neq;
-	if (c->scheme) {
+	if (c->scheme && !c->fp32) {
 		HIPCHECK(hipMemcpyAsync(wn, c->w[c->cur], sizeof(double) * len,
 					hipMemcpyDeviceToHost, c->stream));
 	} else {
-		k_soa_to_aos<<<nblocks(len, 256), 256, 0, c->stream>>>(
-			c->stage, state_ref(c), (long)len);
+		if (c->scheme)
+			k_widen<<<nblocks(len, 256), 256, 0, c->stream>>>(
+				(const float *)c->w[c->cur], c->stage, (long)len);
+		else if (c->fp32)
+			k_soa_to_aos<float><<<nblocks(len, 256), 256, 0, c->stream>>>(
+				c->stage, state_ref<float>(c), (long)len);
+		else
+			k_soa_to_aos<double><<<nblocks(len, 256), 256, 0, c->stream>>>(
+				c->stage, state_ref<double>(c), (long)len);
 		HIPCHECK(hipGetLastError());
 		HIPCHECK(hipMemcpyAsync(wn, c->stage, sizeof(double) * len,
 					hipMemcpyDeviceToHost, c->stream));
@@ -340,14 +386,15 @@ static const double *stage_hostvals(gdh_ctx *c, const double *h, size_t n)
 
 /* Write the macro state wv (6 values) of DoF ipg into the state, as w
  * itself (RK3, SoA) or as f_eq(w) (kinetic, reference layout). */
-__device__ inline void store_dof(const StateRef &s, double *kin, int scheme,
+template <typename T>
+__device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 				 const KinPar &kp, int neq, long ipg,
 				 const double wv[6])
 {
 	if (!scheme) {
 #pragma unroll
 		for (int q = 0; q < 6; q++)
-			*macro_ptr(s, ipg, q) = wv[q];
+			*macro_ptr(s, ipg, q) = (T)wv[q];
 		return;
 	}
 	double qm[6][3];
@@ -355,12 +402,13 @@ __device__ inline void store_dof(const StateRef &s, double *kin, int scheme,
 	for (int iw = 0; iw < 6; iw++)
 		for (int v = 0; v < kp.nv; v++)
 			kin[(long)(iw * kp.nv + v) * neq + ipg] =
-				feq_comp(kp, wv, qm, iw, v);
+				(T)feq_comp(kp, wv, qm, iw, v);
 }
 
 /* n DoFs starting at ipg0 with coordinates x[3 k]; exact values from the
  * closed form of ip, or from hv[6 k] when hv != NULL */
-__global__ void k_fill_exact(StateRef s, double *kin, int scheme, KinPar kp,
+template <typename T>
+__global__ void k_fill_exact(StateRef<T> s, T *kin, int scheme, KinPar kp,
 			     ImposedPar ip, int neq, long ipg0, long n,
 			     const double *__restrict__ x,
 			     const double *__restrict__ hv, double t)
@@ -379,6 +427,20 @@ __global__ void k_fill_exact(StateRef s, double *kin, int scheme, KinPar kp,
 	store_dof(s, kin, scheme, kp, neq, ipg0 + k, wv);
 }
 
+static void fill_launch(gdh_ctx *c, long ipg0, long n, const double *x,
+			const double *hv, double t)
+{
+	if (c->fp32)
+		k_fill_exact<float><<<nblocks(n, 256), 256, 0, c->stream>>>(
+			state_ref<float>(c), (float *)c->w[c->cur], c->scheme, c->kp, c->ip,
+			c->neq, ipg0, n, x, hv, t);
+	else
+		k_fill_exact<double><<<nblocks(n, 256), 256, 0, c->stream>>>(
+			state_ref<double>(c), (double *)c->w[c->cur], c->scheme, c->kp,
+			c->ip, c->neq, ipg0, n, x, hv, t);
+	HIPCHECK(hipGetLastError());
+}
+
 extern "C" void gdh_init_solution(gdh_ctx *c, const double *wexact)
 {
 	const long n = 10L * c->ne;
@@ -387,10 +449,7 @@ extern "C" void gdh_init_solution(gdh_ctx *c, const double *wexact)
 		abort();
 	}
 	const double *hv = stage_hostvals(c, wexact, 6 * n);
-	k_fill_exact<<<nblocks(n, 256), 256, 0, c->stream>>>(
-		state_ref(c), c->w[c->cur], c->scheme, c->kp, c->ip, c->neq, 0, n,
-		c->xnode, hv, 0.0);
-	HIPCHECK(hipGetLastError());
+	fill_launch(c, 0, n, c->xnode, hv, 0.0);
 }
 
 extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
@@ -404,10 +463,7 @@ extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 	}
 	const double *hv = stage_hostvals(c, wghost, 6 * n);
 	prof_begin(c, 2);
-	k_fill_exact<<<nblocks(n, 256), 256, 0, c->stream>>>(
-		state_ref(c), c->w[c->cur], c->scheme, c->kp, c->ip, c->neq,
-		10L * c->ne, n, c->xghost, hv, t);
-	HIPCHECK(hipGetLastError());
+	fill_launch(c, 10L * c->ne, n, c->xghost, hv, t);
 	prof_end(c, 2);
 }
 
@@ -419,8 +475,9 @@ extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 
 /* per-block partial sums of (w - w_exact)^2 wpg_i det_e per component
  * (src/simulation/simulation.c:624-650) */
+template <typename T>
 __global__ __launch_bounds__(ERR_BS) void k_err_partial(
-	const double *__restrict__ st, StateRef sr, int scheme, KinPar kp,
+	const T *__restrict__ st, StateRef<T> sr, int scheme, KinPar kp,
 	ImposedPar ip,
 	const double *__restrict__ geom, const double *__restrict__ xnode,
 	const double *__restrict__ hv, int ne, int neq, double t,
@@ -444,12 +501,12 @@ __global__ __launch_bounds__(ERR_BS) void k_err_partial(
 			}
 			if (!scheme) {
 				for (int q = 0; q < 6; q++)
-					wl[q] = *macro_ptr(sr, ipg, q);
+					wl[q] = (double)*macro_ptr(sr, ipg, q);
 			} else {
 				for (int q = 0; q < 6; q++) {
 					double s = 0.0;
 					for (int v = 0; v < kp.nv; v++)
-						s += st[(long)(q * kp.nv + v) * neq + ipg];
+						s += (double)st[(long)(q * kp.nv + v) * neq + ipg];
 					wl[q] = s;
 				}
 			}
@@ -472,20 +529,32 @@ __global__ __launch_bounds__(ERR_BS) void k_err_partial(
 		partials[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-/* fixed-order final sum: mean over components of sqrt|sum| */
-__global__ void k_err_final(const double *__restrict__ partials, int np,
-			    double *__restrict__ out)
+/* fixed-order final sum: mean over components of sqrt|sum|; 64 lanes take
+ * the partials in a fixed stride, then a fixed-order tree */
+__global__ __launch_bounds__(64) void k_err_final(const double *__restrict__ partials,
+						  int np, double *__restrict__ out)
 {
-	if (threadIdx.x != 0 || blockIdx.x != 0)
-		return;
-	double err = 0.0;
+	__shared__ double red[6][64];
+	const int t = threadIdx.x;
 	for (int q = 0; q < 6; q++) {
 		double s = 0.0;
-		for (int b = 0; b < np; b++)
+		for (int b = t; b < np; b += 64)
 			s += partials[6 * b + q];
-		err += sqrt(fabs(s)) / 6;
+		red[q][t] = s;
 	}
-	out[0] = err;
+	__syncthreads();
+	for (int h = 32; h > 0; h >>= 1) {
+		if (t < h)
+			for (int q = 0; q < 6; q++)
+				red[q][t] += red[q][t + h];
+		__syncthreads();
+	}
+	if (t == 0) {
+		double err = 0.0;
+		for (int q = 0; q < 6; q++)
+			err += sqrt(fabs(red[q][0])) / 6;
+		out[0] = err;
+	}
 }
 
 extern "C" double gdh_error_l2(gdh_ctx *c, double t, const double *wexact)
@@ -496,9 +565,14 @@ extern "C" double gdh_error_l2(gdh_ctx *c, double t, const double *wexact)
 	}
 	const double *hv = stage_hostvals(c, wexact, 60L * c->ne);
 	const int np = std::min(c->npartials, std::max(1, nblocks(c->ne, ERR_BS)));
-	k_err_partial<<<np, ERR_BS, 0, c->stream>>>(
-		c->w[c->cur], state_ref(c), c->scheme, c->kp, c->ip, c->geom, c->xnode, hv,
-		c->ne, c->neq, t, c->partials);
+	if (c->fp32)
+		k_err_partial<float><<<np, ERR_BS, 0, c->stream>>>(
+			(const float *)c->w[c->cur], state_ref<float>(c), c->scheme, c->kp,
+			c->ip, c->geom, c->xnode, hv, c->ne, c->neq, t, c->partials);
+	else
+		k_err_partial<double><<<np, ERR_BS, 0, c->stream>>>(
+			(const double *)c->w[c->cur], state_ref<double>(c), c->scheme,
+			c->kp, c->ip, c->geom, c->xnode, hv, c->ne, c->neq, t, c->partials);
 	HIPCHECK(hipGetLastError());
 	k_err_final<<<1, 64, 0, c->stream>>>(c->partials, np,
 					     c->partials + 6L * c->npartials);

@@ -75,27 +75,28 @@ __device__ inline void face_normal(const double ct[9], int f, double vn[3])
 /* ------------------------------------------------------------- fluxes */
 
 /* Upwind Maxwell flux (src/models/maxwell.c:152-192) with precomputed
- * overr = 1/(|n| + 1e-16). */
-__device__ inline void maxwell_upwind(const double wL[6], const double wR[6],
-				      const double n[3], double overr,
-				      double fl[6])
+ * overr = 1/(|n| + 1e-16); T = double (the plain form) or float (the FP32
+ * form), same operation order. */
+template <typename T>
+__device__ inline void maxwell_upwind(const T wL[6], const T wR[6], const T n[3],
+				      T overr, T fl[6])
 {
-	double avg[6], jmp[6];
+	T avg[6], jmp[6];
 #pragma unroll
 	for (int q = 0; q < 6; q++) {
-		avg[q] = 0.5 * (wR[q] + wL[q]);
-		jmp[q] = 0.5 * (wR[q] - wL[q]);
+		avg[q] = T(0.5) * (wR[q] + wL[q]);
+		jmp[q] = T(0.5) * (wR[q] - wL[q]);
 	}
-	const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+	const T nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
 #pragma unroll
 	for (int blk = 0; blk < 2; blk++) {
-		const double *u = jmp + 3 * blk;
-		const double *c = avg + 3 * (1 - blk);
-		const double ndu = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
-		const double cr0 = n[1] * c[2] - n[2] * c[1];
-		const double cr1 = n[2] * c[0] - n[0] * c[2];
-		const double cr2 = n[0] * c[1] - n[1] * c[0];
-		const double sg = blk == 0 ? -1.0 : 1.0;
+		const T *u = jmp + 3 * blk;
+		const T *c = avg + 3 * (1 - blk);
+		const T ndu = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+		const T cr0 = n[1] * c[2] - n[2] * c[1];
+		const T cr1 = n[2] * c[0] - n[0] * c[2];
+		const T cr2 = n[0] * c[1] - n[1] * c[0];
+		const T sg = blk == 0 ? T(-1) : T(1);
 		fl[3 * blk + 0] = sg * cr0 + overr * (n[0] * ndu - nn * u[0]);
 		fl[3 * blk + 1] = sg * cr1 + overr * (n[1] * ndu - nn * u[1]);
 		fl[3 * blk + 2] = sg * cr2 + overr * (n[2] * ndu - nn * u[2]);
@@ -103,13 +104,13 @@ __device__ inline void maxwell_upwind(const double wL[6], const double wR[6],
 }
 
 /* (-n x H_L, 0) (src/models/maxwell.c:205-223) */
-__device__ inline void maxwell_metal(const double wL[6], const double n[3],
-				     double fl[6])
+template <typename T>
+__device__ inline void maxwell_metal(const T wL[6], const T n[3], T fl[6])
 {
 	fl[0] = n[2] * wL[4] - n[1] * wL[5];
 	fl[1] = n[0] * wL[5] - n[2] * wL[3];
 	fl[2] = n[1] * wL[3] - n[0] * wL[4];
-	fl[3] = fl[4] = fl[5] = 0.0;
+	fl[3] = fl[4] = fl[5] = T(0);
 }
 
 /* ----------------------------------------------------- exact solutions */

@@ -24,13 +24,15 @@ using namespace gdh;
 
 /* The operator tables reach the kernel through a pointer, so that they are
  * read with scalar loads into SGPR operands instead of being materialised as
- * FP64 immediates held in vector registers. */
-struct Rk3Tables {
-	double minv_int[3][10][10];
-	double minv_lift[4][10][6];
+ * immediates held in vector registers. T = double (plain form) or float
+ * (FP32 form: the exact rationals rounded once more, to binary32). */
+template <typename T> struct Rk3Tables {
+	T minv_int[3][10][10];
+	T minv_lift[4][10][6];
 };
-static const Rk3Tables h_rk3_tables = { T10_MINV_INTERNAL_INIT, T10_MINV_LIFT_INIT };
-static Rk3Tables *d_rk3_tables = nullptr;
+static const Rk3Tables<double> h_rk3_tables = { T10_MINV_INTERNAL_INIT, T10_MINV_LIFT_INIT };
+static Rk3Tables<double> *d_rk3_tables64 = nullptr;
+static Rk3Tables<float> *d_rk3_tables32 = nullptr;
 
 #define RK3_BS 128
 
@@ -47,7 +49,7 @@ __device__ inline int xcd_tile(int hw, int nblk)
  * not merged with earlier loads of the same addresses. Re-reading the
  * element's nodes from the L1/L2 is cheaper than keeping all of them live
  * in registers across the kernel. */
-__device__ __forceinline__ const double *reload_ptr(const double *p)
+template <typename T> __device__ __forceinline__ const T *reload_ptr(const T *p)
 {
 	asm volatile("" : "+s"(p));
 	return p;
@@ -56,7 +58,7 @@ __device__ __forceinline__ const double *reload_ptr(const double *p)
 /* Materialise the accumulators at a phase boundary: without it the
  * optimiser re-associates the sums across phases and sinks every product to
  * the final update, keeping all intermediate values live. */
-__device__ __forceinline__ void pin(double (&acc)[6][10])
+template <typename T> __device__ __forceinline__ void pin(T (&acc)[6][10])
 {
 #pragma unroll
 	for (int p = 0; p < 6; p++)
@@ -67,16 +69,15 @@ __device__ __forceinline__ void pin(double (&acc)[6][10])
 }
 
 /* acc_p -= A(n)[p][q] h for the two outputs p that field q feeds */
-template <int Q>
-__device__ __forceinline__ void curl_accumulate(double (&acc)[6][10],
-						const double (&h)[10], double nx,
-						double ny, double nz)
+template <int Q, typename T>
+__device__ __forceinline__ void curl_accumulate(T (&acc)[6][10], const T (&h)[10],
+						T nx, T ny, T nz)
 {
 	/* A(n): E-flux = -n x H, H-flux = n x E */
 	constexpr int p1 = Q == 0 ? 4 : Q == 1 ? 3 : Q == 2 ? 3 : Q == 3 ? 1 : Q == 4 ? 0 : 0;
 	constexpr int p2 = Q == 0 ? 5 : Q == 1 ? 5 : Q == 2 ? 4 : Q == 3 ? 2 : Q == 4 ? 2 : 1;
-	const double c1 = Q == 0 ? nz : Q == 1 ? -nz : Q == 2 ? ny : Q == 3 ? -nz : Q == 4 ? nz : -ny;
-	const double c2 = Q == 0 ? -ny : Q == 1 ? nx : Q == 2 ? -nx : Q == 3 ? ny : Q == 4 ? -nx : nx;
+	const T c1 = Q == 0 ? nz : Q == 1 ? -nz : Q == 2 ? ny : Q == 3 ? -nz : Q == 4 ? nz : -ny;
+	const T c2 = Q == 0 ? -ny : Q == 1 ? nx : Q == 2 ? -nx : Q == 3 ? ny : Q == 4 ? -nx : nx;
 #pragma unroll
 	for (int m = 0; m < 10; m++) {
 		acc[p1][m] -= c1 * h[m];
@@ -84,48 +85,50 @@ __device__ __forceinline__ void curl_accumulate(double (&acc)[6][10],
 	}
 }
 
-template <int Q>
-__device__ __forceinline__ void volume_field(double (&acc)[6][10],
-					     const double *__restrict__ W, long ne,
+template <int Q, typename T>
+__device__ __forceinline__ void volume_field(T (&acc)[6][10],
+					     const T *__restrict__ W, long ne,
 					     int e, const double (&ct)[9],
-					     const Rk3Tables *__restrict__ T)
+					     const Rk3Tables<T> *__restrict__ Tb)
 {
-	double wq[10];
+	T wq[10];
 #pragma unroll
 	for (int j = 0; j < 10; j++)
 		wq[j] = W[(Q * 10 + j) * ne + e];
 #pragma unroll
 	for (int k = 0; k < 3; k++) {
-		double h[10];
+		T h[10];
 #pragma unroll
 		for (int m = 0; m < 10; m++) {
-			double s = 0.0;
+			T s = T(0);
 #pragma unroll
 			for (int j = 0; j < 10; j++)
-				s += T->minv_int[k][m][j] * wq[j];
+				s += Tb->minv_int[k][m][j] * wq[j];
 			h[m] = s;
 		}
-		curl_accumulate<Q>(acc, h, ct[0 + k], ct[3 + k], ct[6 + k]);
+		curl_accumulate<Q>(acc, h, (T)ct[0 + k], (T)ct[3 + k], (T)ct[6 + k]);
 		pin(acc);
 	}
 }
 
-template <int F>
-__device__ __forceinline__ void face_term(double (&acc)[6][10],
-					  const double *__restrict__ W,
-					  const double *__restrict__ GH, long ne,
+/* Face f's contribution; the scaled normal and 1/(|n| + 1e-16) are formed
+ * in FP64 from the FP64 geometry, then used in T. */
+template <int F, typename T>
+__device__ __forceinline__ void face_term(T (&acc)[6][10], const T *__restrict__ W,
+					  const T *__restrict__ GH, long ne,
 					  long ngh, int e, int nb, int map,
-					  const double (&ct)[9], int bflux,
-					  const Rk3Tables *__restrict__ T)
+					  const double (&ctd)[9], int bflux,
+					  const Rk3Tables<T> *__restrict__ Tb)
 {
-	double vn[3];
-	face_normal(ct, F, vn);
-	const double overr =
-		1.0 / (sqrt(vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]) + 1e-16);
+	double vnd[3];
+	face_normal(ctd, F, vnd);
+	const T overr = (T)(1.0 / (sqrt(vnd[0] * vnd[0] + vnd[1] * vnd[1] +
+					vnd[2] * vnd[2]) + 1e-16));
+	const T vn[3] = { (T)vnd[0], (T)vnd[1], (T)vnd[2] };
 #pragma unroll
 	for (int b = 0; b < 6; b++) {
 		const int i = face_node(F, b);
-		double wl[6], wr[6], fl[6];
+		T wl[6], wr[6], fl[6];
 #pragma unroll
 		for (int q = 0; q < 6; q++)
 			wl[q] = W[(q * 10 + i) * ne + e];
@@ -146,7 +149,7 @@ __device__ __forceinline__ void face_term(double (&acc)[6][10],
 			maxwell_upwind(wl, wr, vn, overr, fl);
 #pragma unroll
 		for (int m = 0; m < 10; m++) {
-			const double lift = T->minv_lift[F][m][b];
+			const T lift = Tb->minv_lift[F][m][b];
 #pragma unroll
 			for (int p = 0; p < 6; p++)
 				acc[p][m] += lift * fl[p];
@@ -155,85 +158,106 @@ __device__ __forceinline__ void face_term(double (&acc)[6][10],
 	}
 }
 
+/* One stage over storage columns e; T = double or float for the state and
+ * the flux/volume arithmetic, geometry G always FP64. */
+template <typename T>
 __global__ __launch_bounds__(RK3_BS) void k_rk3_soa(
-	const double *__restrict__ W, double *__restrict__ Wo,
-	double *__restrict__ K2, const double *__restrict__ GH,
-	const double *__restrict__ G, const int4 *__restrict__ nbr,
-	const int4 *__restrict__ nbmap, const Rk3Tables *__restrict__ T, int ne_,
-	int ngh_, double Ac, double Bc, double dt, int first, int bflux)
+	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
+	const T *__restrict__ GH, const double *__restrict__ G,
+	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
+	const Rk3Tables<T> *__restrict__ Tb, int ne_, int ngh_, double Ac,
+	double Bc, double dt, int first, int bflux)
 {
 	const long ne = ne_, ngh = ngh_;
 	const int tile = xcd_tile(blockIdx.x, gridDim.x);
 	const int e = tile * RK3_BS + threadIdx.x;
 	if (e >= ne_)
 		return;
-	double ct[9];
+	double ctd[9];
 #pragma unroll
 	for (int k = 0; k < 9; k++)
-		ct[k] = G[k * ne + e];
+		ctd[k] = G[k * ne + e];
 	const double det = G[9 * ne + e];
 
-	double acc[6][10];
+	T acc[6][10];
 #pragma unroll
 	for (int p = 0; p < 6; p++)
 #pragma unroll
 		for (int m = 0; m < 10; m++)
-			acc[p][m] = 0.0;
+			acc[p][m] = T(0);
 
-	volume_field<0>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<0>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
-	volume_field<1>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<1>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
-	volume_field<2>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<2>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
-	volume_field<3>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<3>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
-	volume_field<4>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<4>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
-	volume_field<5>(acc, reload_ptr(W), ne, e, ct, T);
+	volume_field<5>(acc, reload_ptr(W), ne, e, ctd, Tb);
 	pin(acc);
 
 	const int4 n4 = nbr[e], m4 = nbmap[e];
-	face_term<0>(acc, reload_ptr(W), GH, ne, ngh, e, n4.x, m4.x, ct, bflux, T);
+	face_term<0>(acc, reload_ptr(W), GH, ne, ngh, e, n4.x, m4.x, ctd, bflux, Tb);
 	pin(acc);
-	face_term<1>(acc, reload_ptr(W), GH, ne, ngh, e, n4.y, m4.y, ct, bflux, T);
+	face_term<1>(acc, reload_ptr(W), GH, ne, ngh, e, n4.y, m4.y, ctd, bflux, Tb);
 	pin(acc);
-	face_term<2>(acc, reload_ptr(W), GH, ne, ngh, e, n4.z, m4.z, ct, bflux, T);
+	face_term<2>(acc, reload_ptr(W), GH, ne, ngh, e, n4.z, m4.z, ctd, bflux, Tb);
 	pin(acc);
-	face_term<3>(acc, reload_ptr(W), GH, ne, ngh, e, n4.w, m4.w, ct, bflux, T);
+	face_term<3>(acc, reload_ptr(W), GH, ne, ngh, e, n4.w, m4.w, ctd, bflux, Tb);
 	pin(acc);
 
-	const double invdet = 1.0 / det;
-	const double *Wr = reload_ptr(W);
+	const T invdet = (T)(1.0 / det);
+	const T dtT = (T)dt, AcT = (T)Ac, BcT = (T)Bc;
+	const T *Wr = reload_ptr(W);
 #pragma unroll
 	for (int q = 0; q < 6; q++)
 #pragma unroll
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
-			const double d = acc[q][i] * invdet;
-			const double k = first ? -dt * d : Ac * K2[g] - dt * d;
+			const T d = acc[q][i] * invdet;
+			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
 			K2[g] = k;
-			Wo[g] = Wr[g] + Bc * k;
+			Wo[g] = Wr[g] + BcT * k;
 		}
+}
+
+template <typename T>
+static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+		       double dt, int first)
+{
+	k_rk3_soa<T><<<nblocks(c->ne, RK3_BS), RK3_BS, 0, c->stream>>>(
+		(const T *)c->w[c->cur], (T *)c->w[1 - c->cur], (T *)c->k2,
+		(const T *)c->gh, c->geom_soa, c->nbr, c->nbmap, tab, c->ne, 6 * c->nb,
+		Ac, Bc, dt, first, c->md.bflux_kind);
+	HIPCHECK(hipGetLastError());
 }
 
 extern "C" void gdh_rk3_stage(gdh_ctx *c, int stage, double dt)
 {
 	static const double A[3] = { 0., -5. / 9., -153. / 128. };
 	static const double B[3] = { 1. / 3., 15. / 16., 8. / 15. };
-	const int nblk = nblocks(c->ne, RK3_BS);
-	if (!d_rk3_tables) {
-		d_rk3_tables = (Rk3Tables *)dmalloc(sizeof(Rk3Tables));
-		HIPCHECK(hipMemcpy(d_rk3_tables, &h_rk3_tables, sizeof(Rk3Tables),
+	if (!d_rk3_tables64) {
+		d_rk3_tables64 = (Rk3Tables<double> *)dmalloc(sizeof(Rk3Tables<double>));
+		HIPCHECK(hipMemcpy(d_rk3_tables64, &h_rk3_tables, sizeof(Rk3Tables<double>),
+				   hipMemcpyHostToDevice));
+		Rk3Tables<float> t32;
+		for (int k = 0; k < 300; k++)
+			(&t32.minv_int[0][0][0])[k] = (float)(&h_rk3_tables.minv_int[0][0][0])[k];
+		for (int k = 0; k < 240; k++)
+			(&t32.minv_lift[0][0][0])[k] = (float)(&h_rk3_tables.minv_lift[0][0][0])[k];
+		d_rk3_tables32 = (Rk3Tables<float> *)dmalloc(sizeof(Rk3Tables<float>));
+		HIPCHECK(hipMemcpy(d_rk3_tables32, &t32, sizeof(Rk3Tables<float>),
 				   hipMemcpyHostToDevice));
 	}
 	prof_begin(c, 0);
 	c->prof_launches++;
-	k_rk3_soa<<<nblk, RK3_BS, 0, c->stream>>>(
-		c->w[c->cur], c->w[1 - c->cur], c->k2, c->gh, c->geom_soa, c->nbr,
-		c->nbmap, d_rk3_tables, c->ne, 6 * c->nb, A[stage], B[stage], dt, stage == 0,
-		c->md.bflux_kind);
-	HIPCHECK(hipGetLastError());
+	if (c->fp32)
+		rk3_launch<float>(c, d_rk3_tables32, A[stage], B[stage], dt, stage == 0);
+	else
+		rk3_launch<double>(c, d_rk3_tables64, A[stage], B[stage], dt, stage == 0);
 	prof_end(c, 0);
 	c->cur = 1 - c->cur;
 }

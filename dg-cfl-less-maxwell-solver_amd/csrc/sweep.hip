@@ -238,8 +238,9 @@ struct ItemBuf {
  * LDS only. Lanes: (iw, i) = (field, node) for vectors, (r, b) = (row, face
  * node) for the 10x6 matrices G_f.
  */
+template <typename R>
 __device__ inline void sweep_prepare(int it, const ItemDesc &d,
-				     const double *__restrict__ fo,
+				     const R *__restrict__ fo,
 				     const double *__restrict__ dinv, int ne,
 				     int neq, int nv, const SweepTables &T,
 				     ItemBuf &B, double (&Z)[6][10])
@@ -253,7 +254,7 @@ __device__ inline void sweep_prepare(int it, const ItemDesc &d,
 	double fv = 0.0, Dr[10], Dg[10];
 	if (t < 60) {
 		const long bq = (long)(iw * nv + v) * neq;
-		fv = fo[bq + 10L * e + i];
+		fv = (double)fo[bq + 10L * e + i];
 		double y = 0.0;
 #pragma unroll
 		for (int f = 0; f < 4; f++) {
@@ -263,12 +264,13 @@ __device__ inline void sweep_prepare(int it, const ItemDesc &d,
 			if (d.flags & (16 << f)) {
 #pragma unroll
 				for (int b = 0; b < 6; b++)
-					u[b] = fo[bq + 10L * ne + 6L * d.nb[f] + b];
+					u[b] = (double)fo[bq + 10L * ne + 6L * d.nb[f] + b];
 			} else {
 #pragma unroll
 				for (int b = 0; b < 6; b++)
 					u[b] = (1.0 - THETA) *
-					       fo[bq + 10L * d.nb[f] + ((d.map[f] >> (4 * b)) & 15)];
+					       (double)fo[bq + 10L * d.nb[f] +
+							  ((d.map[f] >> (4 * b)) & 15)];
 			}
 			double s = 0.0;
 #pragma unroll
@@ -330,19 +332,59 @@ __device__ inline void ll_store(u64 *p, u64 x)
 	__hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+/* Hand-off words of one value: binary64 as two words (epoch << 32 | high
+ * half), (epoch << 32 | low half); binary32 as one word (epoch << 32 |
+ * bits). */
+template <typename R> struct Handoff;
+template <> struct Handoff<double> {
+	static constexpr int words = 2;
+	__device__ static void put(u64 *p, double x, unsigned epoch)
+	{
+		const u64 bits = (u64)__double_as_longlong(x), tag = (u64)epoch << 32;
+		ll_store(p, tag | (bits >> 32));
+		ll_store(p + 1, tag | (bits & 0xffffffffull));
+	}
+	/* true (and x set) when both words carry the epoch */
+	__device__ static bool get(const u64 *p, unsigned epoch, double &x)
+	{
+		const u64 hi = ll_load(p), lo = ll_load(p + 1);
+		if ((unsigned)(hi >> 32) != epoch || (unsigned)(lo >> 32) != epoch)
+			return false;
+		x = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+		return true;
+	}
+};
+template <> struct Handoff<float> {
+	static constexpr int words = 1;
+	__device__ static void put(u64 *p, float x, unsigned epoch)
+	{
+		ll_store(p, ((u64)epoch << 32) | (u64)__float_as_uint(x));
+	}
+	__device__ static bool get(const u64 *p, unsigned epoch, double &x)
+	{
+		const u64 w = ll_load(p);
+		if ((unsigned)(w >> 32) != epoch)
+			return false;
+		x = (double)__uint_as_float((unsigned)(w & 0xffffffffull));
+		return true;
+	}
+};
+
 /*
- * Lane s (< nslot) checks one word of slot s's neighbour trace (field 0,
- * face node 0, high half): the full trace is read again only once every
- * slot shows the current epoch there, so that waiting blocks do not flood
- * the memory side with coherent loads.
+ * Lane s (< nslot) checks the first word of one value of slot s's
+ * neighbour trace (field 0, face node 0): the full trace is read again only
+ * once every slot shows the current epoch there, so that waiting blocks do
+ * not flood the memory side with coherent loads.
  */
+template <typename R>
 __device__ inline bool ll_sentinels(const ItemBuf &B, const u64 *ll, int ne,
 				    unsigned epoch)
 {
 	const int t = threadIdx.x;
 	bool wait = false;
 	if (t < B.nslot) {
-		const u64 *p = ll + 2 * (((long)B.v * ne + B.nb[t]) * 60 + (B.map[t] & 15));
+		const u64 *p = ll + Handoff<R>::words *
+					    (((long)B.v * ne + B.nb[t]) * 60 + (B.map[t] & 15));
 		wait = (unsigned)(ll_load(p) >> 32) != epoch;
 	}
 	return __ballot(wait) == 0;
@@ -353,6 +395,7 @@ __device__ inline bool ll_sentinels(const ItemBuf &B, const u64 *ll, int ne,
  * (bit s of pending): each complete one goes to X[s][q][b]. Returns the
  * slots still pending.
  */
+template <typename R>
 __device__ inline unsigned ll_poll(const ItemBuf &B, const u64 *ll, int ne,
 				   unsigned epoch, unsigned pending,
 				   double (&X)[3][6][6])
@@ -364,11 +407,11 @@ __device__ inline unsigned ll_poll(const ItemBuf &B, const u64 *ll, int ne,
 		if (!(pending & (1u << s)))
 			continue;
 		const int j = (B.map[s] >> (4 * b)) & 15;
-		const u64 *p = ll + 2 * (((long)B.v * ne + B.nb[s]) * 60 + q * 10 + j);
-		const u64 hi = ll_load(p), lo = ll_load(p + 1);
-		if ((unsigned)(hi >> 32) == epoch && (unsigned)(lo >> 32) == epoch) {
-			X[s][q][b] = __longlong_as_double(
-				(long long)((hi << 32) | (lo & 0xffffffffull)));
+		const u64 *p = ll + Handoff<R>::words *
+					    (((long)B.v * ne + B.nb[s]) * 60 + q * 10 + j);
+		double x;
+		if (Handoff<R>::get(p, epoch, x)) {
+			X[s][q][b] = x;
 			pending &= ~(1u << s);
 		}
 	}
@@ -390,11 +433,12 @@ __device__ inline bool spin_expired(int spins, int *status)
 	return false;
 }
 
-/* sol = base - sum_s theta G_s sol_nb; published to fn (and ll) */
-template <bool PERSIST>
+/* sol = base - sum_s theta G_s sol_nb; published to fn (and ll), rounded
+ * once to R */
+template <bool PERSIST, typename R>
 __device__ inline void sweep_finish(const ItemBuf &B, const double (*X)[6][6],
-				    double *__restrict__ fn, u64 *ll, int nv,
-				    int ne, int neq, unsigned epoch)
+				    R *__restrict__ fn, u64 *ll, int nv, int ne,
+				    int neq, unsigned epoch)
 {
 	const int t = threadIdx.x;
 	const int iw = t / 10, i = t % 10;
@@ -414,17 +458,14 @@ __device__ inline void sweep_finish(const ItemBuf &B, const double (*X)[6][6],
 #pragma unroll
 				for (int b = 0; b < 6; b++)
 					s -= B.G[sl][i][b] *
-					     fn[g0 + ((B.map[sl] >> (4 * b)) & 15)];
+					     (double)fn[g0 + ((B.map[sl] >> (4 * b)) & 15)];
 			}
 		}
-		fn[bq + 10L * B.e + i] = s;
-		if (PERSIST) {
-			const u64 bits = (u64)__double_as_longlong(s);
-			const u64 tag = (u64)epoch << 32;
-			u64 *p = ll + 2 * (((long)B.v * ne + B.e) * 60 + t);
-			ll_store(p, tag | (bits >> 32));
-			ll_store(p + 1, tag | (bits & 0xffffffffull));
-		}
+		const R sr = (R)s;
+		fn[bq + 10L * B.e + i] = sr;
+		if (PERSIST)
+			Handoff<R>::put(ll + Handoff<R>::words * (((long)B.v * ne + B.e) * 60 + t),
+					sr, epoch);
 	}
 	__syncthreads();
 }
@@ -437,9 +478,10 @@ __device__ inline void load_tables(SweepTables &T)
 }
 
 /* One upwind level per launch, one item per 64-lane block. */
+template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_level(
-	const ItemDesc *__restrict__ desc, int begin, const double *__restrict__ fo,
-	double *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
+	const ItemDesc *__restrict__ desc, int begin, const R *__restrict__ fo,
+	R *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
 	int neq)
 {
 	__shared__ SweepTables T;
@@ -449,7 +491,7 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	const int it = begin + blockIdx.x;
 	const ItemDesc d = desc[it];
 	sweep_prepare(it, d, fo, dinv, ne, neq, nv, T, B, Z);
-	sweep_finish<false>(B, nullptr, fn, nullptr, nv, ne, neq, 0);
+	sweep_finish<false, R>(B, nullptr, fn, nullptr, nv, ne, neq, 0);
 }
 
 /*
@@ -464,9 +506,10 @@ __global__ __launch_bounds__(64) void k_sweep_level(
  * records a 100 us busy interval, and the largest number of overlapping
  * intervals is a set of blocks that were resident together.
  */
+template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_persistent(
-	const ItemDesc *__restrict__ desc, int nitems, const double *__restrict__ fo,
-	double *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
+	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
+	R *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
 	int neq, u64 *ll, unsigned epoch, int *status, u64 *disc)
 {
 	__shared__ SweepTables T;
@@ -497,7 +540,7 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 		/* one full pass; if something is missing, prepare the next item
 		 * and then watch one word per slot */
 		unsigned pending = t < 36 ? (1u << B[k].nslot) - 1u : 0u;
-		pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+		pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 		if (__ballot(pending != 0)) {
 			bool prepared = false;
 			if (next < nitems) {
@@ -505,26 +548,26 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 				prepared = true;
 			}
 			int spins = 0;
-			bool ready = ll_sentinels(B[k], ll, ne, epoch);
+			bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
 			while (!ready) {
 				__builtin_amdgcn_s_sleep(2);
-				ready = ll_sentinels(B[k], ll, ne, epoch) ||
+				ready = ll_sentinels<R>(B[k], ll, ne, epoch) ||
 					spin_expired(++spins, status);
 			}
-			pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+			pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 			while (__ballot(pending != 0)) {
 				__builtin_amdgcn_s_sleep(1);
-				pending = ll_poll(B[k], ll, ne, epoch, pending, X);
+				pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 				if (spin_expired(++spins, status))
 					break;
 			}
 			__syncthreads(); /* X complete */
-			sweep_finish<true>(B[k], X, fn, ll, nv, ne, neq, epoch);
+			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
 			if (next < nitems && !prepared)
 				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 		} else {
 			__syncthreads(); /* X complete */
-			sweep_finish<true>(B[k], X, fn, ll, nv, ne, neq, epoch);
+			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
 			if (next < nitems)
 				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 		}
@@ -537,16 +580,16 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
  * reports 20 blocks per CU for 8000 B of LDS where 18 fit), and a
  * cooperative launch sized by it does not have all its blocks resident.
  */
-static int sweep_residency(gdh_ctx *c)
+template <typename R> static int sweep_residency(gdh_ctx *c)
 {
 	int dev = 0, ncu = 0, per_cu = 0;
 	HIPCHECK(hipGetDevice(&dev));
 	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent,
+	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<R>,
 							      64, 0));
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
-	k_sweep_persistent<<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
+	k_sweep_persistent<R><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
 						     nullptr, 0, 0, 0, nullptr, 0,
 						     nullptr, disc);
 	HIPCHECK(hipGetLastError());
@@ -581,19 +624,19 @@ void sweep_setup(gdh_ctx *c)
 		c->items, c->nitems, c->geom, c->nbr, c->nbmap, c->cls, c->kp, c->dt,
 		c->desc);
 	HIPCHECK(hipGetLastError());
-	const size_t llb = sizeof(u64) * 2 * 60 * (size_t)c->nitems;
+	const size_t llb = sizeof(u64) * (c->fp32 ? 1 : 2) * 60 * (size_t)c->nitems;
 	c->ll = (u64 *)dmalloc(llb);
 	HIPCHECK(hipMemsetAsync(c->ll, 0, llb, c->stream));
 	c->status = (int *)dmalloc(sizeof(int));
 	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	c->epoch = 0;
-	c->sweep_grid = sweep_residency(c);
+	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 	const char *mode = getenv("GDN_SWEEP");
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
 }
 
-void sweep_step(gdh_ctx *c, double *fo, double *fn)
+template <typename R> static void sweep_step_t(gdh_ctx *c, R *fo, R *fn)
 {
 	if (c->sweep_mode == 0) {
 		for (int L = 0; L < c->nlevels; L++) {
@@ -601,15 +644,16 @@ void sweep_step(gdh_ctx *c, double *fo, double *fn)
 			if (!n)
 				continue;
 			c->prof_launches++;
-			k_sweep_level<<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, c->dinv,
-							       c->kp.nv, c->ne, c->neq);
+			k_sweep_level<R><<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, c->dinv,
+								  c->kp.nv, c->ne, c->neq);
 		}
 		HIPCHECK(hipGetLastError());
 		return;
 	}
 	if (++c->epoch == 0) {
 		/* 2^32 sweeps: restart the epochs from a zeroed buffer */
-		HIPCHECK(hipMemsetAsync(c->ll, 0, sizeof(u64) * 2 * 60 * (size_t)c->nitems,
+		HIPCHECK(hipMemsetAsync(c->ll, 0,
+					sizeof(u64) * Handoff<R>::words * 60 * (size_t)c->nitems,
 					c->stream));
 		c->epoch = 1;
 	}
@@ -620,18 +664,27 @@ void sweep_step(gdh_ctx *c, double *fo, double *fn)
 	u64 *ll = c->ll, *disc = nullptr;
 	unsigned epoch = c->epoch;
 	int *status = c->status;
-	void *args[] = { &desc, &nitems, &fo, &fn, &dinv, &nv, &ne, &neq, &ll, &epoch,
+	const R *fo_c = fo;
+	void *args[] = { &desc, &nitems, &fo_c, &fn, &dinv, &nv, &ne, &neq, &ll, &epoch,
 			 &status, &disc };
-	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent,
+	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent<R>,
 					    dim3(std::min(c->sweep_grid, nitems)),
 					    dim3(64), args, 0, c->stream));
+}
+
+void sweep_step(gdh_ctx *c, void *fo, void *fn)
+{
+	if (c->fp32)
+		sweep_step_t<float>(c, (float *)fo, (float *)fn);
+	else
+		sweep_step_t<double>(c, (double *)fo, (double *)fn);
 }
 
 /* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [0, n)
  * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54); NV = kp.nv,
  * fixed at compile time so that the node's 6 NV values stay in registers. */
-template <int NV>
-__global__ __launch_bounds__(256) void k_relax(double *__restrict__ f, KinPar kp,
+template <int NV, typename R>
+__global__ __launch_bounds__(256) void k_relax(R *__restrict__ f, KinPar kp,
 					       long n, int neq)
 {
 	const long ipg = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -644,7 +697,7 @@ __global__ __launch_bounds__(256) void k_relax(double *__restrict__ f, KinPar kp
 		double s = 0.0;
 #pragma unroll
 		for (int v = 0; v < NV; v++) {
-			const double x = f[(long)(iw * NV + v) * neq + ipg];
+			const double x = (double)f[(long)(iw * NV + v) * neq + ipg];
 			fo[iw][v] = x;
 			s += x;
 		}
@@ -658,16 +711,16 @@ __global__ __launch_bounds__(256) void k_relax(double *__restrict__ f, KinPar kp
 #pragma unroll
 		for (int v = 0; v < NV; v++)
 			f[(long)(iw * NV + v) * neq + ipg] =
-				om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v];
+				(R)(om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v]);
 }
 
-void relax_launch(gdh_ctx *c, double *f, long n)
+template <typename R> static void relax_launch_t(gdh_ctx *c, R *f, long n)
 {
 	const int nb = nblocks(n, 256);
 	if (c->kp.nv == 4)
-		k_relax<4><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+		k_relax<4, R><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
 	else if (c->kp.nv == 6)
-		k_relax<6><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+		k_relax<6, R><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
 	else {
 		fprintf(stderr, "gdon3d-hip: no relaxation kernel for %d velocities\n",
 			c->kp.nv);
@@ -676,9 +729,17 @@ void relax_launch(gdh_ctx *c, double *f, long n)
 	HIPCHECK(hipGetLastError());
 }
 
+void relax_launch(gdh_ctx *c, void *f, long n)
+{
+	if (c->fp32)
+		relax_launch_t<float>(c, (float *)f, n);
+	else
+		relax_launch_t<double>(c, (double *)f, n);
+}
+
 extern "C" void gdh_relax_step(gdh_ctx *c)
 {
-	double *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
+	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
 	/* Ghost DoFs are identity rows of M1 and M2; the sweep reads them from
 	 * fo and nothing reads the ghost region of fn before the next ghost
 	 * update overwrites it. */

@@ -139,6 +139,7 @@ def load_library(path=LIB_PATH):
         "gdh_copy_calibration": (C.c_double, [C.c_size_t, C.c_int]),
         "gdn_upwind_classes": (C.POINTER(C.c_uint8), [P(GdnMesh), P(GdnModel)]),
         "dg_stepper_create": (C.c_void_p, [P(GdnSimulation)]),
+        "dg_stepper_create_precision": (C.c_void_p, [P(GdnSimulation), C.c_int]),
         "dg_stepper_init": (None, [C.c_void_p]),
         "dg_stepper_advance": (None, [C.c_void_p, C.c_int]),
         "dg_stepper_time": (C.c_double, [C.c_void_p]),
@@ -282,10 +283,15 @@ class Simulation:
 class Stepper:
     """dg_stepper_* : set-up, initial condition, k-step advances, timing."""
 
-    def __init__(self, sim):
+    PRECISIONS = {"fp64": 0, "fp32": 1}
+
+    def __init__(self, sim, precision="fp64"):
+        """precision: "fp64" (the plain form) or "fp32" (binary32 fields,
+        binary32 RK3 flux/volume arithmetic; see dg_stepper_create_precision)."""
         self.lib = load_library()
         self.sim = sim
-        self.h = self.lib.dg_stepper_create(C.byref(sim.s))
+        self.h = self.lib.dg_stepper_create_precision(C.byref(sim.s),
+                                                      self.PRECISIONS[precision])
         self.lib.dg_stepper_init(self.h)
 
     def advance(self, k):

@@ -307,12 +307,17 @@ double *gdn_eval_imposed_ghosts(const gdn_simulation *simu, double t)
 
 gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md)
 {
+	return gdn_ctx_open_ex(simu, md, GDH_FP64);
+}
+
+gdh_ctx *gdn_ctx_open_ex(gdn_simulation *simu, gdh_model_desc *md, int precision)
+{
 	if (gdh_device_count() < 1)
 		gdn_fatal("no HIP device is visible; the GPU solver cannot run");
 	gdn_model_describe(simu->mdl, simu->use_kinetic_scheme, md);
 	gdh_mesh_desc d;
 	gdn_mesh_describe(simu->tt, &d);
-	gdh_ctx *ctx = gdh_create(&d, md, simu->use_kinetic_scheme ? 1 : 0);
+	gdh_ctx *ctx = gdh_create_ex(&d, md, simu->use_kinetic_scheme ? 1 : 0, precision);
 	gdn_mesh_desc_release(&d);
 	gdh_upload_state(ctx, simu->wn);
 	return ctx;

@@ -38,6 +38,8 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, const uint8_t *cls,
 /* A device context for the simulation with its state uploaded (and the
  * imposed-data arrays of GDH_IMPOSED_HOST helpers). */
 gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md);
+/* the same with the device state precision (GDH_FP64 or GDH_FP32) */
+gdh_ctx *gdn_ctx_open_ex(gdn_simulation *simu, gdh_model_desc *md, int precision);
 
 /* Exact solution from a host callback at the 10 nelem element nodes
  * (nb_w each) or at the 6 nboundaryfaces ghost nodes. Caller frees. */

@@ -32,11 +32,17 @@ static void ghosts(gdn_stepper *st, double t)
 
 gdn_stepper *dg_stepper_create(gdn_simulation *simu)
 {
+	return dg_stepper_create_precision(simu, GDN_PRECISION_FP64);
+}
+
+gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision)
+{
 	gdn_stepper *st = calloc(1, sizeof(*st));
 	if (!st)
 		gdn_fatal("out of memory");
 	st->simu = simu;
-	st->ctx = gdn_ctx_open(simu, &st->md);
+	st->ctx = gdn_ctx_open_ex(simu, &st->md,
+				  precision == GDN_PRECISION_FP32 ? GDH_FP32 : GDH_FP64);
 	if (simu->use_kinetic_scheme) {
 		const gdn_mesh *msh = simu->tt;
 		const int ne = msh->nbelems, nv = simu->mdl->nb_v;

@@ -23,6 +23,16 @@ extern "C" {
 typedef struct gdn_stepper gdn_stepper;
 
 gdn_stepper *dg_stepper_create(gdn_simulation *simu);
+
+/* The same with a choice of precision. GDN_PRECISION_FP64 is the plain form
+ * (dg_stepper_create). GDN_PRECISION_FP32 stores the fields in binary32 and
+ * runs the RK3 flux and volume arithmetic in binary32; geometry, transport
+ * block factors, the transport solve arithmetic, relaxation arithmetic and
+ * the L2 reductions stay binary64. simu->wn stays binary64 (converted at
+ * upload and at dg_stepper_finish). */
+#define GDN_PRECISION_FP64 0
+#define GDN_PRECISION_FP32 1
+gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision);
 void dg_stepper_init(gdn_stepper *st);
 void dg_stepper_advance(gdn_stepper *st, int nsteps);
 gdn_real dg_stepper_time(const gdn_stepper *st);

@@ -84,9 +84,24 @@ typedef struct gdh_ctx gdh_ctx;
 /* Number of visible GPUs (0 when none); does not create a context. */
 int gdh_device_count(void);
 
-/* Upload the mesh and the model; scheme 0 = RK3 (macro), 1 = kinetic. */
+/* Upload the mesh and the model; scheme 0 = RK3 (macro), 1 = kinetic.
+ * Replaces the set-up half of dg_solve_rk3 / dg_solve_relaxation
+ * (src/dg/explicit.c:25-37 assemble_rk3, src/dg/relaxation.c:24-91
+ * assemble_theta_scheme): the operators are applied matrix-free. */
 gdh_ctx *gdh_create(const gdh_mesh_desc *mesh, const gdh_model_desc *model,
 		    int scheme);
+
+/* Precision of the device state (gdh_create_ex). GDH_FP64: the plain form,
+ * matching the reference's binary64 path. GDH_FP32: fields stored in
+ * binary32 and the RK3 flux / volume arithmetic done in binary32; the
+ * geometry, the transport block factors, the transport and relaxation
+ * arithmetic and the L2 reductions stay in binary64.
+ * No counterpart in the reference (gdn_real is fixed at compile time,
+ * src/gdon3d.h). */
+#define GDH_FP64 0
+#define GDH_FP32 1
+gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh, const gdh_model_desc *model,
+		       int scheme, int precision);
 void gdh_destroy(gdh_ctx *ctx);
 
 /* State transfer in the reference layout of simu->wn (m * neq doubles). */
