@@ -74,6 +74,24 @@ struct gdh_ctx {
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */
+	long nlagged; /* inflow faces lagged to break upwind cycles */
+	void *wx; /* zero state (set-up of the capacitance matrices) */
+	void *wz; /* lagged values z, scattered into a state-shaped buffer */
+	int lag_last, lag_max; /* sweeps per step: last step, largest */
+	/* capacitance form over the cycles: distinct lagged nodes (ipg and
+	 * velocity), per-velocity offsets, LU of (I - T S) column-major */
+	int cap_qtot;
+	std::vector<int> cap_off_h;
+	int *cap_ipg, *cap_vel, *cap_off, *cap_piv;
+	double *cap_lu, *cap_t;
+	long *cap_luoff;
+	int cap_qmax() const
+	{
+		int m = 0;
+		for (size_t v = 0; v + 1 < cap_off_h.size(); v++)
+			m = std::max(m, cap_off_h[v + 1] - cap_off_h[v]);
+		return m;
+	}
 	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
 
 	/* optional event timing of the main kernels: pairs of events recorded
@@ -166,7 +184,7 @@ __device__ inline T *macro_ptr(const StateRef<T> &s, long ipg, int q)
 
 /* launchers implemented in rk3.hip and sweep.hip */
 void sweep_setup(gdh_ctx *c);
-void sweep_step(gdh_ctx *c, void *fo, void *fn);
+void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl);
 void relax_launch(gdh_ctx *c, void *f, long n);
 void block_inverse_launch(gdh_ctx *c);
 

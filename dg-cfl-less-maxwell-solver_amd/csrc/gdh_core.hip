@@ -198,6 +198,14 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->cls = nullptr;
 	c->desc = nullptr;
 	c->ll = nullptr;
+	c->wx = nullptr;
+	c->wz = nullptr;
+	c->cap_qtot = 0;
+	c->cap_ipg = c->cap_vel = c->cap_off = c->cap_piv = nullptr;
+	c->cap_lu = c->cap_t = nullptr;
+	c->cap_luoff = nullptr;
+	c->nlagged = 0;
+	c->lag_last = c->lag_max = 0;
 	c->status = nullptr;
 	c->nlevels = 0;
 	c->nitems = 0;
@@ -213,6 +221,8 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->xghost, c->w[0], c->w[1],	 c->k2,
 			 c->hostvals, c->partials, c->items, c->dinv,
 			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa, c->elem_pos,
+			 c->wx, c->wz, c->cap_ipg, c->cap_vel, c->cap_off, c->cap_piv,
+			 c->cap_lu, c->cap_t, c->cap_luoff,
 			 c->stage };
 	for (void *p : ptrs)
 		if (p)

@@ -12,6 +12,9 @@ using namespace gdh;
 /* ===================================================================== */
 
 #define THETA 0.5 /* Crank-Nicolson, src/dg/relaxation.c:133 */
+#define CAP_MAX_NODES 4096 /* lagged nodes of one velocity (dense LU) */
+
+static void cap_setup(gdh_ctx *c);
 
 /* own outward flux coefficient s_f = v . vn(e, f) */
 __device__ inline double face_coef(const double ct[9], int f, const double *v)
@@ -141,7 +144,17 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 	k_block_inverse<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
 		c->items, c->nitems, c->geom, c->cls, c->kp, dt, c->dinv);
 	HIPCHECK(hipGetLastError());
+	/* lagged (cycle-breaking) inflow faces: capacitance form */
+	c->nlagged = 0;
+	for (size_t k = 0; k < (size_t)ne * 4 * nv; k++)
+		c->nlagged += cls[k] == 3;
+	c->lag_last = c->lag_max = 0;
 	sweep_setup(c);
+	if (c->nlagged) {
+		const size_t blen = (size_t)c->m * c->neq;
+		c->wx = dmalloc(c->rsz * blen);
+		cap_setup(c);
+	}
 }
 
 /* ------------------------------------------------------------------ */
@@ -163,13 +176,15 @@ typedef unsigned long long u64;
 struct ItemDesc {
 	double cf[4]; /* s_f dt / det on the inflow faces, 0 elsewhere */
 	int e, v;
-	int flags; /* bit f: inflow face f; bit 4 + f: inflow from the ghosts */
+	int flags; /* bit f: inflow face f; bit 4 + f: inflow from the ghosts;
+		    * bit 8 + f: lagged inflow (cycle broken; value given in xl) */
 	int pad;
 	int nb[4]; /* neighbour element, or the boundary face index */
 	int map[4]; /* 4 bits per face node: the neighbour's local node */
 };
 
 #define SWEEP_SPIN_LIMIT (1 << 20) /* polls before a wait is declared stuck */
+
 
 __global__ void k_item_desc(const int2 *__restrict__ items, int nitems,
 			    const double *__restrict__ geom,
@@ -195,9 +210,11 @@ __global__ void k_item_desc(const int2 *__restrict__ items, int nitems,
 	d.flags = 0;
 	d.pad = 0;
 	for (int f = 0; f < 4; f++) {
-		const bool in = cls[(4L * e + f) * kp.nv + v] == 2;
+		const uint8_t cl = cls[(4L * e + f) * kp.nv + v];
+		const bool in = cl == 2 || cl == 3;
 		d.cf[f] = in ? face_coef(ct, f, kp.vi + 3 * v) * sc : 0.0;
 		d.flags |= in ? (nbv[f] >= 0 ? 1 << f : (1 << f) | (1 << (4 + f))) : 0;
+		d.flags |= cl == 3 ? 1 << (8 + f) : 0;
 		d.nb[f] = nbv[f] >= 0 ? nbv[f] : -1 - nbv[f];
 		d.map[f] = mpv[f];
 	}
@@ -240,7 +257,7 @@ struct ItemBuf {
  */
 template <typename R>
 __device__ inline void sweep_prepare(int it, const ItemDesc &d,
-				     const R *__restrict__ fo,
+				     const R *__restrict__ fo, const R *__restrict__ xl,
 				     const double *__restrict__ dinv, int ne,
 				     int neq, int nv, const SweepTables &T,
 				     ItemBuf &B, double (&Z)[6][10])
@@ -265,6 +282,15 @@ __device__ inline void sweep_prepare(int it, const ItemDesc &d,
 #pragma unroll
 				for (int b = 0; b < 6; b++)
 					u[b] = (double)fo[bq + 10L * ne + 6L * d.nb[f] + b];
+			} else if (d.flags & (256 << f)) {
+				/* lagged: both halves of the trace known (the new
+				 * values from xl; none when xl is null) */
+#pragma unroll
+				for (int b = 0; b < 6; b++) {
+					const long g = bq + 10L * d.nb[f] + ((d.map[f] >> (4 * b)) & 15);
+					u[b] = (1.0 - THETA) * (double)fo[g] +
+					       (xl ? THETA * (double)xl[g] : 0.0);
+				}
 			} else {
 #pragma unroll
 				for (int b = 0; b < 6; b++)
@@ -288,7 +314,7 @@ __device__ inline void sweep_prepare(int it, const ItemDesc &d,
 		int slot = 0;
 #pragma unroll
 		for (int f = 0; f < 4; f++) {
-			if ((d.flags & ~(d.flags >> 4) & (1 << f)) == 0)
+			if ((d.flags & ~(d.flags >> 4) & ~(d.flags >> 8) & (1 << f)) == 0)
 				continue;
 			double s = 0.0;
 #pragma unroll
@@ -303,7 +329,7 @@ __device__ inline void sweep_prepare(int it, const ItemDesc &d,
 		B.v = v;
 		B.e = e;
 		for (int f = 0; f < 4; f++) {
-			if ((d.flags & ~(d.flags >> 4) & (1 << f)) == 0)
+			if ((d.flags & ~(d.flags >> 4) & ~(d.flags >> 8) & (1 << f)) == 0)
 				continue;
 			B.nb[slot] = d.nb[f];
 			B.map[slot] = d.map[f];
@@ -481,8 +507,8 @@ __device__ inline void load_tables(SweepTables &T)
 template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_level(
 	const ItemDesc *__restrict__ desc, int begin, const R *__restrict__ fo,
-	R *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
-	int neq)
+	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
+	int nv, int ne, int neq)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B;
@@ -490,7 +516,7 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	load_tables(T);
 	const int it = begin + blockIdx.x;
 	const ItemDesc d = desc[it];
-	sweep_prepare(it, d, fo, dinv, ne, neq, nv, T, B, Z);
+	sweep_prepare(it, d, fo, xl, dinv, ne, neq, nv, T, B, Z);
 	sweep_finish<false, R>(B, nullptr, fn, nullptr, nv, ne, neq, 0);
 }
 
@@ -509,8 +535,8 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
-	R *__restrict__ fn, const double *__restrict__ dinv, int nv, int ne,
-	int neq, u64 *ll, unsigned epoch, int *status, u64 *disc)
+	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
+	int nv, int ne, int neq, u64 *ll, unsigned epoch, int *status, u64 *disc)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
@@ -531,7 +557,7 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 	load_tables(T);
 	int it = blockIdx.x, k = 0;
 	if (it < nitems)
-		sweep_prepare(it, desc[it], fo, dinv, ne, neq, nv, T, B[0], Z);
+		sweep_prepare(it, desc[it], fo, xl, dinv, ne, neq, nv, T, B[0], Z);
 	for (; it < nitems; it += gridDim.x, k ^= 1) {
 		const int next = it + gridDim.x;
 		ItemDesc dn;
@@ -544,7 +570,7 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 		if (__ballot(pending != 0)) {
 			bool prepared = false;
 			if (next < nitems) {
-				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 				prepared = true;
 			}
 			int spins = 0;
@@ -564,12 +590,12 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 			__syncthreads(); /* X complete */
 			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
 			if (next < nitems && !prepared)
-				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 		} else {
 			__syncthreads(); /* X complete */
 			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
 			if (next < nitems)
-				sweep_prepare(next, dn, fo, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
 		}
 	}
 }
@@ -590,8 +616,8 @@ template <typename R> static int sweep_residency(gdh_ctx *c)
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
 	k_sweep_persistent<R><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
-						     nullptr, 0, 0, 0, nullptr, 0,
-						     nullptr, disc);
+							nullptr, nullptr, 0, 0, 0, nullptr, 0,
+							nullptr, disc);
 	HIPCHECK(hipGetLastError());
 	std::vector<u64> h(2 * (size_t)g0);
 	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(),
@@ -636,7 +662,8 @@ void sweep_setup(gdh_ctx *c)
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
 }
 
-template <typename R> static void sweep_step_t(gdh_ctx *c, R *fo, R *fn)
+template <typename R>
+static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 {
 	if (c->sweep_mode == 0) {
 		for (int L = 0; L < c->nlevels; L++) {
@@ -644,8 +671,9 @@ template <typename R> static void sweep_step_t(gdh_ctx *c, R *fo, R *fn)
 			if (!n)
 				continue;
 			c->prof_launches++;
-			k_sweep_level<R><<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, c->dinv,
-								  c->kp.nv, c->ne, c->neq);
+			k_sweep_level<R><<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, xl,
+								  c->dinv, c->kp.nv, c->ne,
+								  c->neq);
 		}
 		HIPCHECK(hipGetLastError());
 		return;
@@ -665,19 +693,19 @@ template <typename R> static void sweep_step_t(gdh_ctx *c, R *fo, R *fn)
 	unsigned epoch = c->epoch;
 	int *status = c->status;
 	const R *fo_c = fo;
-	void *args[] = { &desc, &nitems, &fo_c, &fn, &dinv, &nv, &ne, &neq, &ll, &epoch,
-			 &status, &disc };
+	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
+			 &epoch, &status, &disc };
 	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent<R>,
 					    dim3(std::min(c->sweep_grid, nitems)),
 					    dim3(64), args, 0, c->stream));
 }
 
-void sweep_step(gdh_ctx *c, void *fo, void *fn)
+void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl)
 {
 	if (c->fp32)
-		sweep_step_t<float>(c, (float *)fo, (float *)fn);
+		sweep_step_t<float>(c, (float *)fo, (float *)fn, (const float *)xl);
 	else
-		sweep_step_t<double>(c, (double *)fo, (double *)fn);
+		sweep_step_t<double>(c, (double *)fo, (double *)fn, (const double *)xl);
 }
 
 /* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [0, n)
@@ -737,6 +765,245 @@ void relax_launch(gdh_ctx *c, void *f, long n)
 		relax_launch_t<double>(c, (double *)f, n);
 }
 
+/*
+ * Exact transport over upwind cycles (capacitance form). The sweep with
+ * lagged inflow values z is affine in (f, z): x = sweep(f, z) = x0 + S z with
+ * x0 = sweep(f, 0). The lagged values are the new solution at the lagged
+ * producers' face nodes, z = T x, so (I - T S) z = T x0: a small dense system
+ * per velocity (q = distinct lagged nodes), the same for the 6 fields. S is
+ * built once at set-up from sweeps with unit lagged inputs (6 columns per
+ * sweep, one per field slot) and (I - T S) is LU-factored on the host in
+ * binary64. Each step: x0 = sweep(f, 0); t = T x0; z = (I - T S)^-1 t;
+ * x = sweep(f, z) — two sweeps, exact up to rounding.
+ */
+template <typename R>
+__global__ void k_cap_gather(const R *__restrict__ x, const int *__restrict__ ipg,
+			     const int *__restrict__ vel, int q, int nv, int neq,
+			     double *__restrict__ t)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 6 * q)
+		return;
+	const int iw = k / q, r = k % q;
+	t[k] = (double)x[(long)(iw * nv + vel[r]) * neq + ipg[r]];
+}
+
+/* z = (I - T S)_v^-1 t for one (v, iw) per block: LU column-major with row
+ * pivots; the result is scattered into the lagged-value state buffer */
+template <typename R>
+__global__ __launch_bounds__(64) void k_cap_solve(
+	const double *__restrict__ lu, const long *__restrict__ luoff,
+	const int *__restrict__ piv, const int *__restrict__ off,
+	const int *__restrict__ ipg, const double *__restrict__ t, int qtot, int nv,
+	int neq, R *__restrict__ wz)
+{
+	extern __shared__ double y[];
+	const int v = blockIdx.x / 6, iw = blockIdx.x % 6;
+	const int o = off[v], q = off[v + 1] - o;
+	const double *A = lu + luoff[v];
+	const int *p = piv + o;
+	const double *tv = t + (long)iw * qtot + o;
+	for (int i = threadIdx.x; i < q; i += 64)
+		y[i] = tv[p[i]];
+	__syncthreads();
+	for (int j = 0; j < q; j++) { /* unit lower triangle */
+		const double yj = y[j];
+		for (int i = j + 1 + threadIdx.x; i < q; i += 64)
+			y[i] -= A[(long)j * q + i] * yj;
+		__syncthreads();
+	}
+	for (int j = q - 1; j >= 0; j--) { /* upper triangle */
+		if (threadIdx.x == 0)
+			y[j] /= A[(long)j * q + j];
+		__syncthreads();
+		const double yj = y[j];
+		for (int i = threadIdx.x; i < j; i += 64)
+			y[i] -= A[(long)j * q + i] * yj;
+		__syncthreads();
+	}
+	for (int i = threadIdx.x; i < q; i += 64)
+		wz[(long)(iw * nv + v) * neq + ipg[o + i]] = (R)y[i];
+}
+
+/* unit lagged inputs of set-up batch c: field slot iw carries column
+ * 6 c + iw of every velocity */
+template <typename R>
+__global__ void k_cap_unit(R *__restrict__ wz, const int *__restrict__ ipg,
+			   const int *__restrict__ off, int batch, int nv, int neq)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 6 * nv)
+		return;
+	const int v = k / 6, iw = k % 6, r = 6 * batch + iw;
+	if (r < off[v + 1] - off[v])
+		wz[(long)(iw * nv + v) * neq + ipg[off[v] + r]] = (R)1;
+}
+
+static void cap_gather(gdh_ctx *c, const void *x)
+{
+	const int q = c->cap_qtot;
+	if (c->fp32)
+		k_cap_gather<float><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
+			(const float *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->neq, c->cap_t);
+	else
+		k_cap_gather<double><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
+			(const double *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->neq, c->cap_t);
+	HIPCHECK(hipGetLastError());
+}
+
+/* partial-pivoting LU of the n x n column-major matrix a, in place */
+static void lu_factor(double *a, int n, int *piv)
+{
+	for (int i = 0; i < n; i++)
+		piv[i] = i;
+	for (int k = 0; k < n; k++) {
+		int p = k;
+		for (int i = k + 1; i < n; i++)
+			if (fabs(a[(long)k * n + i]) > fabs(a[(long)k * n + p]))
+				p = i;
+		if (a[(long)k * n + p] == 0.0) {
+			fprintf(stderr, "gdon3d-hip: singular capacitance matrix of the "
+					"transport over upwind cycles\n");
+			abort();
+		}
+		if (p != k) {
+			for (int j = 0; j < n; j++)
+				std::swap(a[(long)j * n + k], a[(long)j * n + p]);
+			std::swap(piv[k], piv[p]);
+		}
+		const double d = a[(long)k * n + k];
+		for (int i = k + 1; i < n; i++)
+			a[(long)k * n + i] /= d;
+		for (int j = k + 1; j < n; j++) {
+			const double akj = a[(long)j * n + k];
+			if (akj == 0.0)
+				continue;
+			for (int i = k + 1; i < n; i++)
+				a[(long)j * n + i] -= a[(long)k * n + i] * akj;
+		}
+	}
+}
+
+/* Set-up of the capacitance form: positions, S by unit sweeps, LU. */
+static void cap_setup(gdh_ctx *c)
+{
+	const int nv = c->kp.nv;
+	/* distinct lagged producer nodes per velocity, from the item descriptors */
+	std::vector<ItemDesc> desc(c->nitems);
+	HIPCHECK(hipMemcpy(desc.data(), c->desc, sizeof(ItemDesc) * desc.size(),
+			   hipMemcpyDeviceToHost));
+	std::vector<std::vector<int>> pos(nv);
+	for (const ItemDesc &d : desc)
+		for (int f = 0; f < 4; f++)
+			if (d.flags & (256 << f))
+				for (int b = 0; b < 6; b++)
+					pos[d.v].push_back(10 * d.nb[f] + ((d.map[f] >> (4 * b)) & 15));
+	std::vector<int> off(nv + 1, 0), ipg, vel;
+	for (int v = 0; v < nv; v++) {
+		std::sort(pos[v].begin(), pos[v].end());
+		pos[v].erase(std::unique(pos[v].begin(), pos[v].end()), pos[v].end());
+		off[v + 1] = off[v] + (int)pos[v].size();
+		for (int x : pos[v]) {
+			ipg.push_back(x);
+			vel.push_back(v);
+		}
+	}
+	const int qtot = off[nv];
+	int qmax = 0;
+	for (int v = 0; v < nv; v++)
+		qmax = std::max(qmax, off[v + 1] - off[v]);
+	if (qmax > CAP_MAX_NODES) {
+		fprintf(stderr, "gdon3d-hip: %d lagged nodes in one velocity's upwind "
+				"cycles (limit %d)\n", qmax, CAP_MAX_NODES);
+		abort();
+	}
+	c->cap_qtot = qtot;
+	c->cap_off_h = off;
+	c->cap_ipg = (int *)dmalloc(sizeof(int) * qtot);
+	c->cap_vel = (int *)dmalloc(sizeof(int) * qtot);
+	c->cap_off = (int *)dmalloc(sizeof(int) * (nv + 1));
+	HIPCHECK(hipMemcpy(c->cap_ipg, ipg.data(), sizeof(int) * qtot, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(c->cap_vel, vel.data(), sizeof(int) * qtot, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(c->cap_off, off.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice));
+	c->cap_t = (double *)dmalloc(sizeof(double) * 6 * qtot);
+
+	/* S columns: sweeps of zero data with unit lagged inputs */
+	const size_t blen = (size_t)c->m * c->neq;
+	c->wz = dmalloc(c->rsz * blen);
+	void *zero = c->wx, *out = c->w[1 - c->cur];
+	HIPCHECK(hipMemsetAsync(zero, 0, c->rsz * blen, c->stream));
+	std::vector<std::vector<double>> M(nv);
+	for (int v = 0; v < nv; v++) {
+		const int q = off[v + 1] - off[v];
+		M[v].assign((size_t)q * q, 0.0);
+		for (int i = 0; i < q; i++)
+			M[v][(size_t)i * q + i] = 1.0;
+	}
+	std::vector<double> th(6 * (size_t)qtot);
+	const int nbatch = (qmax + 5) / 6;
+	for (int bt = 0; bt < nbatch; bt++) {
+		HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
+		if (c->fp32)
+			k_cap_unit<float><<<nblocks(6 * nv, 64), 64, 0, c->stream>>>(
+				(float *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->neq);
+		else
+			k_cap_unit<double><<<nblocks(6 * nv, 64), 64, 0, c->stream>>>(
+				(double *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->neq);
+		HIPCHECK(hipGetLastError());
+		sweep_step(c, zero, out, c->wz);
+		cap_gather(c, out);
+		HIPCHECK(hipMemcpyAsync(th.data(), c->cap_t, sizeof(double) * th.size(),
+					hipMemcpyDeviceToHost, c->stream));
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		for (int v = 0; v < nv; v++) {
+			const int q = off[v + 1] - off[v];
+			for (int iw = 0; iw < 6; iw++) {
+				const int col = 6 * bt + iw;
+				if (col >= q)
+					continue;
+				for (int i = 0; i < q; i++) /* column-major */
+					M[v][(size_t)col * q + i] -= th[(size_t)iw * qtot + off[v] + i];
+			}
+		}
+	}
+	HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
+	std::vector<long> luoff(nv + 1, 0);
+	std::vector<int> piv(qtot > 0 ? qtot : 1);
+	for (int v = 0; v < nv; v++) {
+		const int q = off[v + 1] - off[v];
+		lu_factor(M[v].data(), q, piv.data() + off[v]);
+		luoff[v + 1] = luoff[v] + (long)q * q;
+	}
+	c->cap_lu = (double *)dmalloc(sizeof(double) * (luoff[nv] > 0 ? luoff[nv] : 1));
+	for (int v = 0; v < nv; v++)
+		if (luoff[v + 1] > luoff[v])
+			HIPCHECK(hipMemcpy(c->cap_lu + luoff[v], M[v].data(),
+					   sizeof(double) * (luoff[v + 1] - luoff[v]),
+					   hipMemcpyHostToDevice));
+	c->cap_luoff = (long *)dmalloc(sizeof(long) * (nv + 1));
+	HIPCHECK(hipMemcpy(c->cap_luoff, luoff.data(), sizeof(long) * (nv + 1),
+			   hipMemcpyHostToDevice));
+	c->cap_piv = (int *)dmalloc(sizeof(int) * piv.size());
+	HIPCHECK(hipMemcpy(c->cap_piv, piv.data(), sizeof(int) * piv.size(),
+			   hipMemcpyHostToDevice));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+static void cap_solve(gdh_ctx *c)
+{
+	const int nv = c->kp.nv;
+	const size_t lds = sizeof(double) * std::max(1, c->cap_qmax());
+	if (c->fp32)
+		k_cap_solve<float><<<6 * nv, 64, lds, c->stream>>>(
+			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
+			c->cap_qtot, nv, c->neq, (float *)c->wz);
+	else
+		k_cap_solve<double><<<6 * nv, 64, lds, c->stream>>>(
+			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
+			c->cap_qtot, nv, c->neq, (double *)c->wz);
+	HIPCHECK(hipGetLastError());
+}
+
 extern "C" void gdh_relax_step(gdh_ctx *c)
 {
 	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
@@ -744,7 +1011,17 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 	 * fo and nothing reads the ghost region of fn before the next ghost
 	 * update overwrites it. */
 	prof_begin(c, 0);
-	sweep_step(c, fo, fn);
+	if (!c->nlagged) {
+		sweep_step(c, fo, fn, nullptr);
+	} else {
+		/* exact over the upwind cycles: two sweeps and a small solve */
+		sweep_step(c, fo, fn, nullptr);
+		cap_gather(c, fn);
+		cap_solve(c);
+		sweep_step(c, fo, fn, c->wz);
+		c->lag_last = 2;
+		c->lag_max = 2;
+	}
 	prof_end(c, 0);
 	const long n = 10L * c->ne;
 	prof_begin(c, 1);
@@ -760,3 +1037,10 @@ extern "C" void gdh_apply_relaxation(gdh_ctx *c)
 	relax_launch(c, c->w[c->cur], n);
 }
 
+
+extern "C" void gdh_transport_stats(gdh_ctx *c, int out[3])
+{
+	out[0] = c->nlagged;
+	out[1] = c->lag_last;
+	out[2] = c->lag_max;
+}

@@ -150,6 +150,7 @@ def load_library(path=LIB_PATH):
         "dg_stepper_finish": (None, [C.c_void_p]),
         "dg_stepper_profile": (None, [C.c_void_p, C.c_int, c_double_p]),
         "dg_stepper_free": (None, [C.c_void_p]),
+        "dg_stepper_transport_stats": (None, [C.c_void_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
         "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
         "t10_get_inv_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
@@ -315,6 +316,12 @@ class Stepper:
         out = np.zeros(4)
         self.lib.dg_stepper_profile(self.h, int(enable), dptr(out))
         return out
+
+    def transport_stats(self):
+        """(lagged inflow faces, sweeps of the last step, most sweeps in a step)"""
+        out = (C.c_int * 3)()
+        self.lib.dg_stepper_transport_stats(self.h, out)
+        return tuple(out)
 
     def finish(self):
         self.lib.dg_stepper_finish(self.h)

@@ -211,74 +211,140 @@ uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl)
 	return cls;
 }
 
-int gdn_sweep_schedule(int nelem, int nv, const int *nbr, const uint8_t *cls,
-		       int *items, int *level_ptr, int *nlevels)
+/* Kahn's algorithm over the inflow (class 2) edges of velocity v: lev[e]
+ * = longest upwind path; returns the number of elements placed (nelem when
+ * the graph is acyclic). */
+static int kahn_levels(int nelem, int nv, int v, const int *nbr,
+		       const uint8_t *cls, int *indeg, int *queue, int *lev)
 {
-	int *indeg = malloc(sizeof(int) * (size_t)(nelem ? nelem : 1));
-	int *queue = malloc(sizeof(int) * (size_t)(nelem ? nelem : 1));
-	int *level = malloc(sizeof(int) * (size_t)nv * (nelem ? nelem : 1));
-	if (!indeg || !queue || !level)
-		gdn_fatal("out of memory");
-	int maxlev = -1, rc = 0;
-	for (int v = 0; v < nv && !rc; v++) {
-		int *lev = level + (size_t)v * nelem;
-		for (int e = 0; e < nelem; e++) {
-			int d = 0;
-			for (int f = 0; f < 4; f++)
-				d += cls[((size_t)e * 4 + f) * nv + v] == 2 &&
-				     nbr[4 * e + f] >= 0;
-			indeg[e] = d;
-			lev[e] = 0;
+	for (int e = 0; e < nelem; e++) {
+		int d = 0;
+		for (int f = 0; f < 4; f++)
+			d += cls[((size_t)e * 4 + f) * nv + v] == 2 && nbr[4 * e + f] >= 0;
+		indeg[e] = d;
+		lev[e] = 0;
+	}
+	int head = 0, tail = 0;
+	for (int e = 0; e < nelem; e++)
+		if (!indeg[e])
+			queue[tail++] = e;
+	while (head < tail) {
+		const int e = queue[head++];
+		for (int f = 0; f < 4; f++) {
+			const int nb = nbr[4 * e + f];
+			if (nb < 0 || cls[((size_t)e * 4 + f) * nv + v] != 1)
+				continue;
+			/* the consumer sees e through an inflow face unless lagged */
+			int g = 0;
+			while (g < 4 && nbr[4 * nb + g] != e)
+				g++;
+			if (g == 4 || cls[((size_t)nb * 4 + g) * nv + v] != 2)
+				continue;
+			if (lev[e] + 1 > lev[nb])
+				lev[nb] = lev[e] + 1;
+			if (--indeg[nb] == 0)
+				queue[tail++] = nb;
 		}
-		int head = 0, tail = 0;
-		for (int e = 0; e < nelem; e++)
-			if (!indeg[e])
-				queue[tail++] = e;
-		while (head < tail) {
-			const int e = queue[head++];
-			for (int f = 0; f < 4; f++) {
-				const int nb = nbr[4 * e + f];
-				if (nb < 0 || cls[((size_t)e * 4 + f) * nv + v] != 1)
-					continue;
-				if (lev[e] + 1 > lev[nb])
-					lev[nb] = lev[e] + 1;
-				if (--indeg[nb] == 0)
-					queue[tail++] = nb;
+	}
+	return tail;
+}
+
+/* Depth-first search over the elements Kahn could not place (the cycles
+ * and what lies downstream of them), following upwind -> downwind edges;
+ * every edge into an element on the current DFS path closes a cycle and is
+ * lagged: the consumer's face class becomes 3. A graph without its DFS
+ * back edges is acyclic. Returns the number of lagged edges. */
+static int lag_back_edges(int nelem, int nv, int v, const int *nbr, uint8_t *cls,
+			  const int *indeg, int *color, int *stack)
+{
+	int nlag = 0;
+	for (int e = 0; e < nelem; e++)
+		color[e] = indeg[e] > 0 ? 0 : 2; /* 0 unvisited, 1 on path, 2 done */
+	for (int r = 0; r < nelem; r++) {
+		if (color[r])
+			continue;
+		int top = 0;
+		stack[2 * top] = r;
+		stack[2 * top + 1] = 0;
+		color[r] = 1;
+		while (top >= 0) {
+			const int e = stack[2 * top];
+			const int f = stack[2 * top + 1]++;
+			if (f == 4) {
+				color[e] = 2;
+				top--;
+				continue;
+			}
+			const int nb = nbr[4 * e + f];
+			if (nb < 0 || cls[((size_t)e * 4 + f) * nv + v] != 1)
+				continue;
+			int g = 0;
+			while (g < 4 && nbr[4 * nb + g] != e)
+				g++;
+			if (g == 4 || cls[((size_t)nb * 4 + g) * nv + v] != 2)
+				continue;
+			if (color[nb] == 1) {
+				cls[((size_t)nb * 4 + g) * nv + v] = 3;
+				nlag++;
+			} else if (color[nb] == 0) {
+				color[nb] = 1;
+				top++;
+				stack[2 * top] = nb;
+				stack[2 * top + 1] = 0;
 			}
 		}
-		if (tail != nelem) {
-			rc = -1 - v;
-			break;
+	}
+	return nlag;
+}
+
+int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
+		       int *items, int *level_ptr, int *nlevels)
+{
+	const size_t n1 = nelem ? nelem : 1;
+	int *indeg = malloc(sizeof(int) * n1);
+	int *queue = malloc(sizeof(int) * n1);
+	int *color = malloc(sizeof(int) * n1);
+	int *stack = malloc(sizeof(int) * 2 * n1);
+	int *level = malloc(sizeof(int) * (size_t)nv * n1);
+	if (!indeg || !queue || !color || !stack || !level)
+		gdn_fatal("out of memory");
+	int maxlev = -1, nlag = 0;
+	for (int v = 0; v < nv; v++) {
+		int *lev = level + (size_t)v * nelem;
+		if (kahn_levels(nelem, nv, v, nbr, cls, indeg, queue, lev) != nelem) {
+			nlag += lag_back_edges(nelem, nv, v, nbr, cls, indeg, color, stack);
+			if (kahn_levels(nelem, nv, v, nbr, cls, indeg, queue, lev) != nelem)
+				gdn_fatal("upwind graph still cyclic after lagging back edges");
 		}
 		for (int e = 0; e < nelem; e++)
 			if (lev[e] > maxlev)
 				maxlev = lev[e];
 	}
-	if (!rc) {
-		/* counting sort by level; (v, e) increasing inside a level */
-		const int nl = maxlev + 1;
-		for (int L = 0; L <= nl; L++)
-			level_ptr[L] = 0;
-		for (int v = 0; v < nv; v++)
-			for (int e = 0; e < nelem; e++)
-				level_ptr[level[(size_t)v * nelem + e] + 1]++;
-		for (int L = 0; L < nl; L++)
-			level_ptr[L + 1] += level_ptr[L];
-		int *pos = malloc(sizeof(int) * (size_t)(nl ? nl : 1));
-		if (!pos)
-			gdn_fatal("out of memory");
-		for (int L = 0; L < nl; L++)
-			pos[L] = level_ptr[L];
-		for (int v = 0; v < nv; v++)
-			for (int e = 0; e < nelem; e++)
-				items[pos[level[(size_t)v * nelem + e]]++] = v * nelem + e;
-		free(pos);
-		*nlevels = nl;
-	}
+	/* counting sort by level; (v, e) increasing inside a level */
+	const int nl = maxlev + 1;
+	for (int L = 0; L <= nl; L++)
+		level_ptr[L] = 0;
+	for (int v = 0; v < nv; v++)
+		for (int e = 0; e < nelem; e++)
+			level_ptr[level[(size_t)v * nelem + e] + 1]++;
+	for (int L = 0; L < nl; L++)
+		level_ptr[L + 1] += level_ptr[L];
+	int *pos = malloc(sizeof(int) * (size_t)(nl ? nl : 1));
+	if (!pos)
+		gdn_fatal("out of memory");
+	for (int L = 0; L < nl; L++)
+		pos[L] = level_ptr[L];
+	for (int v = 0; v < nv; v++)
+		for (int e = 0; e < nelem; e++)
+			items[pos[level[(size_t)v * nelem + e]]++] = v * nelem + e;
+	free(pos);
+	*nlevels = nl;
 	free(indeg);
 	free(queue);
+	free(color);
+	free(stack);
 	free(level);
-	return rc;
+	return nlag;
 }
 
 double *gdn_eval_imposed_elements(const gdn_simulation *simu, double t)

@@ -29,10 +29,14 @@ uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl);
 /* Level-ordered work list of the upwind transport sweep (longest-path
  * levels of each velocity's upwind graph, Kahn's algorithm). nbr is the
  * 4-per-element neighbour array of gdh_mesh_desc, cls the classes above.
- * Fills items[nv * nelem] with v * nelem + e sorted by (level, v, e) and
- * level_ptr[nlevels + 1]; returns 0, or -1 - v when the graph of velocity v
- * has a cycle (then items is unspecified). */
-int gdn_sweep_schedule(int nelem, int nv, const int *nbr, const uint8_t *cls,
+ * Where a velocity's graph has cycles, the edges that close them (the back
+ * edges of a depth-first search from the elements Kahn cannot place) are
+ * lagged: the consumer's face class 2 becomes 3, the sweep takes that
+ * inflow as a given value, and the device solves for those values exactly
+ * (capacitance form, two sweeps per step). Fills items[nv * nelem] with v * nelem + e sorted by
+ * (level, v, e) and level_ptr[nlevels + 1]; returns the number of lagged
+ * edges (0 for acyclic graphs). */
+int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 		       int *items, int *level_ptr, int *nlevels);
 
 /* A device context for the simulation with its state uploaded (and the

@@ -19,6 +19,7 @@ struct gdn_stepper {
 	double t;
 	int iter;
 	int levels;
+	int lagged; /* upwind edges lagged to break cycles */
 };
 
 static void ghosts(gdn_stepper *st, double t)
@@ -54,15 +55,8 @@ gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision)
 		if (!items || !lptr)
 			gdn_fatal("out of memory");
 		int nl = 0;
-		const int rc = gdn_sweep_schedule(ne, nv, d.nbr, cls, items, lptr, &nl);
+		st->lagged = gdn_sweep_schedule(ne, nv, d.nbr, cls, items, lptr, &nl);
 		gdn_mesh_desc_release(&d);
-		if (rc) {
-			fprintf(stderr,
-				"gdon3d: the upwind graph of velocity %d has a cycle; "
-				"the block-triangular sweep needs an acyclic graph\n",
-				-1 - rc);
-			abort();
-		}
 		st->levels = nl;
 		gdh_relax_setup(st->ctx, simu->dt, cls, items, lptr, nl);
 		free(items);
@@ -155,4 +149,13 @@ void dg_stepper_free(gdn_stepper *st)
 		return;
 	gdh_destroy(st->ctx);
 	free(st);
+}
+
+void dg_stepper_transport_stats(const gdn_stepper *st, int out[3])
+{
+	if (st->simu->use_kinetic_scheme) {
+		gdh_transport_stats(st->ctx, out);
+	} else {
+		out[0] = out[1] = out[2] = 0;
+	}
 }

@@ -51,6 +51,11 @@ void dg_stepper_profile(gdn_stepper *st, int enable, double out[4]);
 /* Number of upwind levels of the worst velocity (kinetic scheme, else 0). */
 int dg_stepper_levels(const gdn_stepper *st);
 
+/* Kinetic scheme on meshes whose upwind graphs have cycles: out[0] lagged
+ * inflow faces, out[1] transport sweeps of the last step, out[2] largest
+ * number of sweeps in one step (gdh_transport_stats). */
+void dg_stepper_transport_stats(const gdn_stepper *st, int out[3]);
+
 #ifdef __cplusplus
 }
 #endif

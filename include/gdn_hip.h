@@ -157,6 +157,17 @@ void gdh_synchronize(gdh_ctx *ctx);
 void gdh_timer_start(gdh_ctx *ctx);
 double gdh_timer_stop(gdh_ctx *ctx);
 
+/* Transport over upwind cycles: out[0] = inflow faces lagged to break the
+ * cycles of the velocities' upwind graphs (0: every graph acyclic, one
+ * sweep per step), out[1] = sweeps of the last step, out[2] = largest number
+ * of sweeps in one step so far. Where the reference factors the whole M1
+ * with KLU (src/maths/sparse.c:411-466), whose block triangular form keeps
+ * each cycle as one block, the GPU sweep lags the edges that close cycles
+ * and solves exactly in capacitance form: x = sweep(f, z) with the lagged
+ * values z from (I - T S) z = T sweep(f, 0), S the sweep's response to unit
+ * lagged inputs (built and LU-factored once at set-up): two sweeps per step. */
+void gdh_transport_stats(gdh_ctx *ctx, int out[3]);
+
 /* Copies `bytes` (read) into a second buffer (written) `reps` times with a
  * plain streaming kernel (k_copy_calib) after one warm-up launch; returns
  * the mean milliseconds per launch. No counterpart in the reference: the

@@ -43,3 +43,25 @@ def cavity_par(lmn=(1, 1, 1), a=(1, -1, 0)):
     par[0:3], par[3:6] = k, a
     par[6] = np.pi * np.sqrt(float(np.dot(lmn, lmn)))
     return par
+
+
+def twisted_msh(n, alpha):
+    """The Kuhn n^3 x 6 cube twisted about the vertical axis through its
+    centre by alpha * z radians (vertices only: the cells stay affine and
+    positively oriented for the values used here). Strong twists make the
+    upwind graphs of fixed directions cyclic, which jitter of a Kuhn mesh
+    does not while its cells stay positively oriented."""
+    src = kuhn_msh(n, 0.0, 0)
+    path = os.path.join(_TMP, f"twist_{n}_{alpha}.msh")
+    if not os.path.exists(path):
+        lines = open(src).read().split("\n")
+        i = lines.index("$Nodes")
+        for k in range(i + 2, i + 2 + int(lines[i + 1])):
+            t = lines[k].split()
+            x, y, z = (float(c) for c in t[1:4])
+            c, s = np.cos(alpha * z), np.sin(alpha * z)
+            u, v = x - 0.5, y - 0.5
+            lines[k] = f"{t[0]} {0.5 + c * u - s * v:.17g} {0.5 + s * u + c * v:.17g} {z:.17g}"
+        with open(path, "w") as f:
+            f.write("\n".join(lines))
+    return path
