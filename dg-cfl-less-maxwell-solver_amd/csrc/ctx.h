@@ -33,6 +33,19 @@ using namespace gdh;
 /*                               context                                 */
 /* ===================================================================== */
 
+/* Instances of a batch (several meshes as one disjoint union): per mesh
+ * element and per boundary face the instance, per instance the exact-
+ * solution parameters, time step and clock; RK3 also per storage column
+ * the instance and 1/eps. ninst == 0: not a batch. Passed by value. */
+struct BatchTab {
+	int ninst;
+	const int *einst, *binst, *einst_col;
+	const double *par; /* GDH_IMPOSED_NPAR per instance */
+	const double *dt;
+	double *time;
+	const double *ieps_col; /* RK3: 1 / eps per storage column, or null */
+};
+
 struct gdh_ctx {
 	int ne, nb, neq, m, scheme;
 	int fp32; /* state stored (and fluxes computed) in binary32 */
@@ -74,6 +87,16 @@ struct gdh_ctx {
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */
+	unsigned *epoch_dev; /* batches: sweep epoch in device memory (graphs) */
+
+	/* batch of instances; graphs of two steps (by state parity) */
+	BatchTab bt;
+	int *bt_mem[4];
+	double *bt_dmem[4];
+	std::vector<double> bt_dt, bt_par;
+	std::vector<int> bt_e0, bt_nb0; /* first element / boundary face per instance */
+	hipGraphExec_t graph[2];
+	int rk3_bs, relax_bs; /* tile sizes (autotuned for batches) */
 	long nlagged; /* inflow faces lagged to break upwind cycles */
 	void *wx; /* zero state (set-up of the capacitance matrices) */
 	void *wz; /* lagged values z, scattered into a state-shaped buffer */

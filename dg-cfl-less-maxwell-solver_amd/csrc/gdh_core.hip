@@ -200,6 +200,15 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->ll = nullptr;
 	c->wx = nullptr;
 	c->wz = nullptr;
+	c->epoch_dev = nullptr;
+	memset(&c->bt, 0, sizeof(c->bt));
+	for (int k = 0; k < 4; k++) {
+		c->bt_mem[k] = nullptr;
+		c->bt_dmem[k] = nullptr;
+	}
+	c->graph[0] = c->graph[1] = nullptr;
+	c->rk3_bs = 128;
+	c->relax_bs = 256;
 	c->cap_qtot = 0;
 	c->cap_ipg = c->cap_vel = c->cap_off = c->cap_piv = nullptr;
 	c->cap_lu = c->cap_t = nullptr;
@@ -222,11 +231,16 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->hostvals, c->partials, c->items, c->dinv,
 			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa, c->elem_pos,
 			 c->wx, c->wz, c->cap_ipg, c->cap_vel, c->cap_off, c->cap_piv,
-			 c->cap_lu, c->cap_t, c->cap_luoff,
+			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev,
+			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
+			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
 			 c->stage };
 	for (void *p : ptrs)
 		if (p)
 			HIPCHECK(hipFree(p));
+	for (int k = 0; k < 2; k++)
+		if (c->graph[k])
+			HIPCHECK(hipGraphExecDestroy(c->graph[k]));
 	HIPCHECK(hipEventDestroy(c->ev0));
 	HIPCHECK(hipEventDestroy(c->ev1));
 	prof_resolve(c);
@@ -415,13 +429,21 @@ __device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 				(T)feq_comp(kp, wv, qm, iw, v);
 }
 
+/* t + c dt rounded as the host loops round it (no fused multiply-add) */
+__device__ inline double stage_time(double t, double c, double dt)
+{
+#pragma clang fp contract(off)
+	return t + c * dt;
+}
+
 /* n DoFs starting at ipg0 with coordinates x[3 k]; exact values from the
  * closed form of ip, or from hv[6 k] when hv != NULL */
 template <typename T>
 __global__ void k_fill_exact(StateRef<T> s, T *kin, int scheme, KinPar kp,
 			     ImposedPar ip, int neq, long ipg0, long n,
 			     const double *__restrict__ x,
-			     const double *__restrict__ hv, double t)
+			     const double *__restrict__ hv, double t, BatchTab bt,
+			     double cstage)
 {
 	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
@@ -432,22 +454,32 @@ __global__ void k_fill_exact(StateRef<T> s, T *kin, int scheme, KinPar kp,
 			wv[q] = hv[6 * k + q];
 	} else {
 		const double xx[3] = { x[3 * k], x[3 * k + 1], x[3 * k + 2] };
-		imposed(ip, xx, t, wv);
+		if (bt.ninst) {
+			/* this DoF's instance: its parameters, clock and step */
+			const long ipg = ipg0 + k, ne10 = 10L * s.ne;
+			const int in = ipg < ne10 ? bt.einst[ipg / 10] : bt.binst[(ipg - ne10) / 6];
+			ImposedPar q = ip;
+			for (int j = 0; j < GDH_IMPOSED_NPAR; j++)
+				q.p[j] = bt.par[GDH_IMPOSED_NPAR * in + j];
+			imposed(q, xx, stage_time(bt.time[in], cstage, bt.dt[in]), wv);
+		} else {
+			imposed(ip, xx, t, wv);
+		}
 	}
 	store_dof(s, kin, scheme, kp, neq, ipg0 + k, wv);
 }
 
 static void fill_launch(gdh_ctx *c, long ipg0, long n, const double *x,
-			const double *hv, double t)
+			const double *hv, double t, double cstage)
 {
 	if (c->fp32)
 		k_fill_exact<float><<<nblocks(n, 256), 256, 0, c->stream>>>(
 			state_ref<float>(c), (float *)c->w[c->cur], c->scheme, c->kp, c->ip,
-			c->neq, ipg0, n, x, hv, t);
+			c->neq, ipg0, n, x, hv, t, c->bt, cstage);
 	else
 		k_fill_exact<double><<<nblocks(n, 256), 256, 0, c->stream>>>(
 			state_ref<double>(c), (double *)c->w[c->cur], c->scheme, c->kp,
-			c->ip, c->neq, ipg0, n, x, hv, t);
+			c->ip, c->neq, ipg0, n, x, hv, t, c->bt, cstage);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -459,7 +491,7 @@ extern "C" void gdh_init_solution(gdh_ctx *c, const double *wexact)
 		abort();
 	}
 	const double *hv = stage_hostvals(c, wexact, 6 * n);
-	fill_launch(c, 0, n, c->xnode, hv, 0.0);
+	fill_launch(c, 0, n, c->xnode, hv, 0.0, 0.0);
 }
 
 extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
@@ -473,8 +505,16 @@ extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 	}
 	const double *hv = stage_hostvals(c, wghost, 6 * n);
 	prof_begin(c, 2);
-	fill_launch(c, 10L * c->ne, n, c->xghost, hv, t);
+	fill_launch(c, 10L * c->ne, n, c->xghost, hv, t, 0.0);
 	prof_end(c, 2);
+}
+
+/* batches: ghosts at each instance's clock + cstage * its time step */
+void ghost_fill_batch(gdh_ctx *c, double cstage)
+{
+	const long n = 6L * c->nb;
+	if (n)
+		fill_launch(c, 10L * c->ne, n, c->xghost, nullptr, 0.0, cstage);
 }
 
 /* ===================================================================== */
@@ -490,12 +530,12 @@ __global__ __launch_bounds__(ERR_BS) void k_err_partial(
 	const T *__restrict__ st, StateRef<T> sr, int scheme, KinPar kp,
 	ImposedPar ip,
 	const double *__restrict__ geom, const double *__restrict__ xnode,
-	const double *__restrict__ hv, int ne, int neq, double t,
+	const double *__restrict__ hv, int e0, int e1, int neq, double t,
 	double *__restrict__ partials)
 {
 	__shared__ double red[6][ERR_BS];
 	double acc[6] = { 0, 0, 0, 0, 0, 0 };
-	for (int e = blockIdx.x * ERR_BS + threadIdx.x; e < ne;
+	for (int e = e0 + blockIdx.x * ERR_BS + threadIdx.x; e < e1;
 	     e += gridDim.x * ERR_BS) {
 		const double det = geom[10L * e + 9];
 		for (int i = 0; i < 10; i++) {
@@ -567,22 +607,20 @@ __global__ __launch_bounds__(64) void k_err_final(const double *__restrict__ par
 	}
 }
 
-extern "C" double gdh_error_l2(gdh_ctx *c, double t, const double *wexact)
+/* L2 error over the elements [e0, e1) against ip at time t (hv: host
+ * exact values of those elements' nodes, or null) */
+double error_range(gdh_ctx *c, int e0, int e1, double t, const ImposedPar &ip,
+		   const double *hv)
 {
-	if (c->ip.kind == GDH_IMPOSED_HOST && !wexact) {
-		fprintf(stderr, "gdon3d-hip: host exact values missing\n");
-		abort();
-	}
-	const double *hv = stage_hostvals(c, wexact, 60L * c->ne);
-	const int np = std::min(c->npartials, std::max(1, nblocks(c->ne, ERR_BS)));
+	const int np = std::min(c->npartials, std::max(1, nblocks(e1 - e0, ERR_BS)));
 	if (c->fp32)
 		k_err_partial<float><<<np, ERR_BS, 0, c->stream>>>(
 			(const float *)c->w[c->cur], state_ref<float>(c), c->scheme, c->kp,
-			c->ip, c->geom, c->xnode, hv, c->ne, c->neq, t, c->partials);
+			ip, c->geom, c->xnode, hv, e0, e1, c->neq, t, c->partials);
 	else
 		k_err_partial<double><<<np, ERR_BS, 0, c->stream>>>(
 			(const double *)c->w[c->cur], state_ref<double>(c), c->scheme,
-			c->kp, c->ip, c->geom, c->xnode, hv, c->ne, c->neq, t, c->partials);
+			c->kp, ip, c->geom, c->xnode, hv, e0, e1, c->neq, t, c->partials);
 	HIPCHECK(hipGetLastError());
 	k_err_final<<<1, 64, 0, c->stream>>>(c->partials, np,
 					     c->partials + 6L * c->npartials);
@@ -592,4 +630,14 @@ extern "C" double gdh_error_l2(gdh_ctx *c, double t, const double *wexact)
 				sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	return err;
+}
+
+extern "C" double gdh_error_l2(gdh_ctx *c, double t, const double *wexact)
+{
+	if (c->ip.kind == GDH_IMPOSED_HOST && !wexact) {
+		fprintf(stderr, "gdon3d-hip: host exact values missing\n");
+		abort();
+	}
+	const double *hv = stage_hostvals(c, wexact, 60L * c->ne);
+	return error_range(c, 0, c->ne, t, c->ip, hv);
 }

@@ -34,7 +34,6 @@ static const Rk3Tables<double> h_rk3_tables = { T10_MINV_INTERNAL_INIT, T10_MINV
 static Rk3Tables<double> *d_rk3_tables64 = nullptr;
 static Rk3Tables<float> *d_rk3_tables32 = nullptr;
 
-#define RK3_BS 128
 
 /* Bijection from the dispatch order of blocks to tiles such that the blocks
  * the hardware sends to one XCD (block ids congruent mod 8) get a contiguous
@@ -159,18 +158,21 @@ __device__ __forceinline__ void face_term(T (&acc)[6][10], const T *__restrict__
 }
 
 /* One stage over storage columns e; T = double or float for the state and
- * the flux/volume arithmetic, geometry G always FP64. */
-template <typename T>
-__global__ __launch_bounds__(RK3_BS) void k_rk3_soa(
+ * the flux/volume arithmetic, geometry G always FP64; BS elements per block.
+ * Batches: the time step of the column's instance (dt_tab[inst[e]]) and,
+ * with ieps, dE/dt scaled by 1/eps of the column (eps dE/dt = curl H). */
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_rk3_soa(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
 	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
 	const Rk3Tables<T> *__restrict__ Tb, int ne_, int ngh_, double Ac,
-	double Bc, double dt, int first, int bflux)
+	double Bc, double dt, int first, int bflux, const int *__restrict__ inst,
+	const double *__restrict__ dt_tab, const double *__restrict__ ieps)
 {
 	const long ne = ne_, ngh = ngh_;
 	const int tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int e = tile * RK3_BS + threadIdx.x;
+	const int e = tile * BS + threadIdx.x;
 	if (e >= ne_)
 		return;
 	double ctd[9];
@@ -210,54 +212,86 @@ __global__ __launch_bounds__(RK3_BS) void k_rk3_soa(
 	pin(acc);
 
 	const T invdet = (T)(1.0 / det);
-	const T dtT = (T)dt, AcT = (T)Ac, BcT = (T)Bc;
+	const T idE = ieps ? (T)(ieps[e] / det) : invdet;
+	const T dtT = (T)(inst ? dt_tab[inst[e]] : dt), AcT = (T)Ac, BcT = (T)Bc;
 	const T *Wr = reload_ptr(W);
 #pragma unroll
 	for (int q = 0; q < 6; q++)
 #pragma unroll
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
-			const T d = acc[q][i] * invdet;
+			const T d = acc[q][i] * (q < 3 ? idE : invdet);
 			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
 			K2[g] = k;
 			Wo[g] = Wr[g] + BcT * k;
 		}
 }
 
+template <typename T, int BS>
+static void rk3_launch_bs(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+			  double dt, int first, const void *W, void *Wo)
+{
+	k_rk3_soa<T, BS><<<nblocks(c->ne, BS), BS, 0, c->stream>>>(
+		(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
+		c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
+		c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
+	HIPCHECK(hipGetLastError());
+}
+
 template <typename T>
 static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
-		       double dt, int first)
+		       double dt, int first, const void *W, void *Wo)
 {
-	k_rk3_soa<T><<<nblocks(c->ne, RK3_BS), RK3_BS, 0, c->stream>>>(
-		(const T *)c->w[c->cur], (T *)c->w[1 - c->cur], (T *)c->k2,
-		(const T *)c->gh, c->geom_soa, c->nbr, c->nbmap, tab, c->ne, 6 * c->nb,
-		Ac, Bc, dt, first, c->md.bflux_kind);
-	HIPCHECK(hipGetLastError());
+	switch (c->rk3_bs) {
+	case 64: rk3_launch_bs<T, 64>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	case 256: rk3_launch_bs<T, 256>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	default: rk3_launch_bs<T, 128>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	}
+}
+
+static void rk3_tables_init()
+{
+	if (d_rk3_tables64)
+		return;
+	d_rk3_tables64 = (Rk3Tables<double> *)dmalloc(sizeof(Rk3Tables<double>));
+	HIPCHECK(hipMemcpy(d_rk3_tables64, &h_rk3_tables, sizeof(Rk3Tables<double>),
+			   hipMemcpyHostToDevice));
+	Rk3Tables<float> t32;
+	for (int k = 0; k < 300; k++)
+		(&t32.minv_int[0][0][0])[k] = (float)(&h_rk3_tables.minv_int[0][0][0])[k];
+	for (int k = 0; k < 240; k++)
+		(&t32.minv_lift[0][0][0])[k] = (float)(&h_rk3_tables.minv_lift[0][0][0])[k];
+	d_rk3_tables32 = (Rk3Tables<float> *)dmalloc(sizeof(Rk3Tables<float>));
+	HIPCHECK(hipMemcpy(d_rk3_tables32, &t32, sizeof(Rk3Tables<float>),
+			   hipMemcpyHostToDevice));
+}
+
+/* One first-stage launch from the current state into the other buffer
+ * (timing only: used by the tile autotuner before a run starts). */
+void rk3_trial(gdh_ctx *c)
+{
+	rk3_tables_init();
+	if (c->fp32)
+		rk3_launch<float>(c, d_rk3_tables32, 0.0, 1.0 / 3.0, 1e-3, 1,
+				  c->w[c->cur], c->w[1 - c->cur]);
+	else
+		rk3_launch<double>(c, d_rk3_tables64, 0.0, 1.0 / 3.0, 1e-3, 1,
+				   c->w[c->cur], c->w[1 - c->cur]);
 }
 
 extern "C" void gdh_rk3_stage(gdh_ctx *c, int stage, double dt)
 {
 	static const double A[3] = { 0., -5. / 9., -153. / 128. };
 	static const double B[3] = { 1. / 3., 15. / 16., 8. / 15. };
-	if (!d_rk3_tables64) {
-		d_rk3_tables64 = (Rk3Tables<double> *)dmalloc(sizeof(Rk3Tables<double>));
-		HIPCHECK(hipMemcpy(d_rk3_tables64, &h_rk3_tables, sizeof(Rk3Tables<double>),
-				   hipMemcpyHostToDevice));
-		Rk3Tables<float> t32;
-		for (int k = 0; k < 300; k++)
-			(&t32.minv_int[0][0][0])[k] = (float)(&h_rk3_tables.minv_int[0][0][0])[k];
-		for (int k = 0; k < 240; k++)
-			(&t32.minv_lift[0][0][0])[k] = (float)(&h_rk3_tables.minv_lift[0][0][0])[k];
-		d_rk3_tables32 = (Rk3Tables<float> *)dmalloc(sizeof(Rk3Tables<float>));
-		HIPCHECK(hipMemcpy(d_rk3_tables32, &t32, sizeof(Rk3Tables<float>),
-				   hipMemcpyHostToDevice));
-	}
+	rk3_tables_init();
 	prof_begin(c, 0);
 	c->prof_launches++;
 	if (c->fp32)
-		rk3_launch<float>(c, d_rk3_tables32, A[stage], B[stage], dt, stage == 0);
+		rk3_launch<float>(c, d_rk3_tables32, A[stage], B[stage], dt, stage == 0,
+				  c->w[c->cur], c->w[1 - c->cur]);
 	else
-		rk3_launch<double>(c, d_rk3_tables64, A[stage], B[stage], dt, stage == 0);
+		rk3_launch<double>(c, d_rk3_tables64, A[stage], B[stage], dt, stage == 0,
+				   c->w[c->cur], c->w[1 - c->cur]);
 	prof_end(c, 0);
 	c->cur = 1 - c->cur;
 }

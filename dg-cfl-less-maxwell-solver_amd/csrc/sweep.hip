@@ -33,12 +33,13 @@ __device__ inline double face_coef(const double ct[9], int f, const double *v)
 __global__ void k_block_inverse(const int2 *__restrict__ items, int nitems,
 				const double *__restrict__ geom,
 				const uint8_t *__restrict__ cls, KinPar kp,
-				double dt, double *__restrict__ dinv)
+				double dt_, double *__restrict__ dinv, BatchTab bt)
 {
 	const int it = blockIdx.x * blockDim.x + threadIdx.x;
 	if (it >= nitems)
 		return;
 	const int v = items[it].x, e = items[it].y;
+	const double dt = bt.ninst ? bt.dt[bt.einst[e]] : dt_;
 	double ct[9];
 	for (int k = 0; k < 9; k++)
 		ct[k] = geom[10L * e + k];
@@ -142,7 +143,7 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 			   hipMemcpyHostToDevice));
 	c->dinv = (double *)dmalloc(sizeof(double) * 100 * items.size());
 	k_block_inverse<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
-		c->items, c->nitems, c->geom, c->cls, c->kp, dt, c->dinv);
+		c->items, c->nitems, c->geom, c->cls, c->kp, dt, c->dinv, c->bt);
 	HIPCHECK(hipGetLastError());
 	/* lagged (cycle-breaking) inflow faces: capacitance form */
 	c->nlagged = 0;
@@ -190,13 +191,14 @@ __global__ void k_item_desc(const int2 *__restrict__ items, int nitems,
 			    const double *__restrict__ geom,
 			    const int4 *__restrict__ nbr,
 			    const int4 *__restrict__ nbmap,
-			    const uint8_t *__restrict__ cls, KinPar kp, double dt,
-			    ItemDesc *__restrict__ desc)
+			    const uint8_t *__restrict__ cls, KinPar kp, double dt_,
+			    ItemDesc *__restrict__ desc, BatchTab bt)
 {
 	const int it = blockIdx.x * blockDim.x + threadIdx.x;
 	if (it >= nitems)
 		return;
 	const int v = items[it].x, e = items[it].y;
+	const double dt = bt.ninst ? bt.dt[bt.einst[e]] : dt_;
 	double ct[9];
 	for (int k = 0; k < 9; k++)
 		ct[k] = geom[10L * e + k];
@@ -536,7 +538,8 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
-	int nv, int ne, int neq, u64 *ll, unsigned epoch, int *status, u64 *disc)
+	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
+	const unsigned *__restrict__ epoch_dev, int *status, u64 *disc)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
@@ -555,6 +558,8 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 		return;
 	}
 	load_tables(T);
+	/* the sweep's epoch: an argument, or (captured graphs) device memory */
+	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
 	int it = blockIdx.x, k = 0;
 	if (it < nitems)
 		sweep_prepare(it, desc[it], fo, xl, dinv, ne, neq, nv, T, B[0], Z);
@@ -617,7 +622,7 @@ template <typename R> static int sweep_residency(gdh_ctx *c)
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
 	k_sweep_persistent<R><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
 							nullptr, nullptr, 0, 0, 0, nullptr, 0,
-							nullptr, disc);
+							nullptr, nullptr, disc);
 	HIPCHECK(hipGetLastError());
 	std::vector<u64> h(2 * (size_t)g0);
 	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(),
@@ -648,7 +653,7 @@ void sweep_setup(gdh_ctx *c)
 	c->desc = (ItemDesc *)dmalloc(sizeof(ItemDesc) * (size_t)c->nitems);
 	k_item_desc<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
 		c->items, c->nitems, c->geom, c->nbr, c->nbmap, c->cls, c->kp, c->dt,
-		c->desc);
+		c->desc, c->bt);
 	HIPCHECK(hipGetLastError());
 	const size_t llb = sizeof(u64) * (c->fp32 ? 1 : 2) * 60 * (size_t)c->nitems;
 	c->ll = (u64 *)dmalloc(llb);
@@ -660,6 +665,11 @@ void sweep_setup(gdh_ctx *c)
 	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 	const char *mode = getenv("GDN_SWEEP");
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
+}
+
+__global__ void k_epoch_bump(unsigned *e)
+{
+	*e = *e + 1u == 0u ? 1u : *e + 1u;
 }
 
 template <typename R>
@@ -678,6 +688,25 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 		HIPCHECK(hipGetLastError());
 		return;
 	}
+	c->prof_launches++;
+	const ItemDesc *desc = c->desc;
+	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
+	const double *dinv = c->dinv;
+	u64 *ll = c->ll, *disc = nullptr;
+	int *status = c->status;
+	const R *fo_c = fo;
+	const int grid = std::min(c->sweep_grid, nitems);
+	if (c->epoch_dev) {
+		/* batches (graph capture): a one-thread node advances the epoch in
+		 * device memory; an ordinary launch with the measured resident
+		 * grid, since a captured stream takes no cooperative launch */
+		k_epoch_bump<<<1, 1, 0, c->stream>>>(c->epoch_dev);
+		k_sweep_persistent<R><<<grid, 64, 0, c->stream>>>(
+			desc, nitems, fo_c, fn, xl, dinv, nv, ne, neq, ll, 0u, c->epoch_dev,
+			status, disc);
+		HIPCHECK(hipGetLastError());
+		return;
+	}
 	if (++c->epoch == 0) {
 		/* 2^32 sweeps: restart the epochs from a zeroed buffer */
 		HIPCHECK(hipMemsetAsync(c->ll, 0,
@@ -685,18 +714,11 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 					c->stream));
 		c->epoch = 1;
 	}
-	c->prof_launches++;
-	const ItemDesc *desc = c->desc;
-	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
-	const double *dinv = c->dinv;
-	u64 *ll = c->ll, *disc = nullptr;
 	unsigned epoch = c->epoch;
-	int *status = c->status;
-	const R *fo_c = fo;
+	const unsigned *edev = nullptr;
 	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
-			 &epoch, &status, &disc };
-	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent<R>,
-					    dim3(std::min(c->sweep_grid, nitems)),
+			 &epoch, &edev, &status, &disc };
+	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent<R>, dim3(grid),
 					    dim3(64), args, 0, c->stream));
 }
 
@@ -711,9 +733,9 @@ void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl)
 /* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [0, n)
  * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54); NV = kp.nv,
  * fixed at compile time so that the node's 6 NV values stay in registers. */
-template <int NV, typename R>
-__global__ __launch_bounds__(256) void k_relax(R *__restrict__ f, KinPar kp,
-					       long n, int neq)
+template <int NV, typename R, int BS>
+__global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp,
+					      long n, int neq)
 {
 	const long ipg = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (ipg >= n)
@@ -742,19 +764,28 @@ __global__ __launch_bounds__(256) void k_relax(R *__restrict__ f, KinPar kp,
 				(R)(om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v]);
 }
 
-template <typename R> static void relax_launch_t(gdh_ctx *c, R *f, long n)
+template <typename R, int BS> static void relax_launch_bs(gdh_ctx *c, R *f, long n)
 {
-	const int nb = nblocks(n, 256);
+	const int nb = nblocks(n, BS);
 	if (c->kp.nv == 4)
-		k_relax<4, R><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+		k_relax<4, R, BS><<<nb, BS, 0, c->stream>>>(f, c->kp, n, c->neq);
 	else if (c->kp.nv == 6)
-		k_relax<6, R><<<nb, 256, 0, c->stream>>>(f, c->kp, n, c->neq);
+		k_relax<6, R, BS><<<nb, BS, 0, c->stream>>>(f, c->kp, n, c->neq);
 	else {
 		fprintf(stderr, "gdon3d-hip: no relaxation kernel for %d velocities\n",
 			c->kp.nv);
 		abort();
 	}
 	HIPCHECK(hipGetLastError());
+}
+
+template <typename R> static void relax_launch_t(gdh_ctx *c, R *f, long n)
+{
+	switch (c->relax_bs) {
+	case 128: relax_launch_bs<R, 128>(c, f, n); break;
+	case 512: relax_launch_bs<R, 512>(c, f, n); break;
+	default: relax_launch_bs<R, 256>(c, f, n); break;
+	}
 }
 
 void relax_launch(gdh_ctx *c, void *f, long n)

@@ -151,6 +151,16 @@ def load_library(path=LIB_PATH):
         "dg_stepper_profile": (None, [C.c_void_p, C.c_int, c_double_p]),
         "dg_stepper_free": (None, [C.c_void_p]),
         "dg_stepper_transport_stats": (None, [C.c_void_p, C.POINTER(C.c_int)]),
+        "dg_batch_create": (C.c_void_p, [C.POINTER(C.POINTER(GdnSimulation)), C.c_int,
+                                         c_double_p, c_double_p, C.c_int]),
+        "dg_batch_tiles": (None, [C.c_void_p, C.POINTER(C.c_int)]),
+        "dg_batch_advance": (None, [C.c_void_p, C.c_int]),
+        "dg_batch_synchronize": (None, [C.c_void_p]),
+        "dg_batch_iterations": (C.c_int, [C.c_void_p]),
+        "dg_batch_errors": (None, [C.c_void_p, c_double_p]),
+        "dg_batch_times": (None, [C.c_void_p, c_double_p]),
+        "dg_batch_finish": (None, [C.c_void_p]),
+        "dg_batch_free": (None, [C.c_void_p]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
         "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
         "t10_get_inv_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
@@ -329,4 +339,52 @@ class Stepper:
     def free(self):
         if self.h:
             self.lib.dg_stepper_free(self.h)
+            self.h = None
+
+
+class Batch:
+    """dg_batch_* : independent simulations advanced together (one device
+    context over the union of their meshes, captured-graph steps,
+    autotuned tiles). par: array (count, 16) of exact-solution parameter
+    blocks; eps: permittivity per element (RK3), instance after instance."""
+
+    def __init__(self, sims, par, eps=None, precision="fp32"):
+        self.lib = load_library()
+        self.sims = list(sims)
+        arr = (C.POINTER(GdnSimulation) * len(self.sims))(
+            *[C.pointer(s.s) for s in self.sims])
+        self._par = np.ascontiguousarray(par, dtype=np.float64).reshape(-1)
+        self._eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+        self.h = self.lib.dg_batch_create(
+            arr, len(self.sims), dptr(self._par),
+            dptr(self._eps) if self._eps is not None else None,
+            Stepper.PRECISIONS[precision])
+
+    def tiles(self):
+        out = (C.c_int * 2)()
+        self.lib.dg_batch_tiles(self.h, out)
+        return tuple(out)
+
+    def advance(self, k):
+        self.lib.dg_batch_advance(self.h, k)
+
+    def sync(self):
+        self.lib.dg_batch_synchronize(self.h)
+
+    def errors(self):
+        out = np.zeros(len(self.sims))
+        self.lib.dg_batch_errors(self.h, dptr(out))
+        return out
+
+    def times(self):
+        out = np.zeros(len(self.sims))
+        self.lib.dg_batch_times(self.h, dptr(out))
+        return out
+
+    def finish(self):
+        self.lib.dg_batch_finish(self.h)
+
+    def free(self):
+        if self.h:
+            self.lib.dg_batch_free(self.h)
             self.h = None

@@ -168,6 +168,31 @@ double gdh_timer_stop(gdh_ctx *ctx);
  * lagged inputs (built and LU-factored once at set-up): two sweeps per step. */
 void gdh_transport_stats(gdh_ctx *ctx, int out[3]);
 
+/* ---- batches (no counterpart in the reference: one drop-in simulation
+ * per call there). The context's mesh is the disjoint union of ninst
+ * instance meshes: instance i owns elements [elem_first[i], elem_first[i+1])
+ * and boundary faces [bface_first[i], bface_first[i+1]), with its own
+ * closed-form exact data (the model's imposed_kind with the parameter block
+ * par[GDH_IMPOSED_NPAR * i ...]), time step dt[i] and clock (starting at 0).
+ * eps: permittivity per element (RK3 only; eps dE/dt = curl H), or NULL.
+ * Call after gdh_create_ex and before gdh_relax_setup (kinetic) and
+ * gdh_init_solution. */
+void gdh_batch_setup(gdh_ctx *ctx, int ninst, const int *elem_first,
+		     const int *bface_first, const double *par, const double *dt,
+		     const double *eps);
+/* Tile sizes by timing candidates on the batch: out[0] elements per block of
+ * the RK3 stage (64/128/256), out[1] nodes per block of the relaxation
+ * (128/256/512). Call before the run starts (it overwrites the second state
+ * buffer). */
+void gdh_batch_autotune(gdh_ctx *ctx, int out[2]);
+/* nsteps time steps of every instance (RK3: ghosts at t + c_s dt and the
+ * stage, three times; kinetic: ghosts at t + dt/2, transport, relaxation),
+ * replayed from a captured HIP graph of two steps. */
+void gdh_batch_advance(gdh_ctx *ctx, int nsteps);
+void gdh_batch_times(gdh_ctx *ctx, double *t);
+void gdh_batch_ghosts(gdh_ctx *ctx);
+void gdh_batch_error_l2(gdh_ctx *ctx, double *err);
+
 /* Copies `bytes` (read) into a second buffer (written) `reps` times with a
  * plain streaming kernel (k_copy_calib) after one warm-up launch; returns
  * the mean milliseconds per launch. No counterpart in the reference: the
