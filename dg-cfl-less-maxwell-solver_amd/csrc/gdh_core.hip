@@ -70,6 +70,21 @@ static std::vector<int> elem_order(const gdh_mesh_desc *mesh)
 	return order;
 }
 
+/*
+ * One stream for every context of the process: the persistent transport
+ * sweep needs all its blocks resident at once, which holds only while no
+ * other kernel shares the device, so contexts do not run concurrently with
+ * each other (each context's kernels fill the device anyway). Created on
+ * first use, never destroyed.
+ */
+static hipStream_t shared_stream()
+{
+	static hipStream_t s = nullptr;
+	if (!s)
+		HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+	return s;
+}
+
 extern "C" gdh_ctx *gdh_create(const gdh_mesh_desc *mesh,
 			       const gdh_model_desc *model, int scheme)
 {
@@ -109,7 +124,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 		fprintf(stderr, "gdon3d-hip: kinetic scheme without velocity set\n");
 		abort();
 	}
-	HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+	c->stream = shared_stream();
 	HIPCHECK(hipEventCreate(&c->ev0));
 	HIPCHECK(hipEventCreate(&c->ev1));
 	c->prof = 0;
@@ -246,7 +261,6 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 	prof_resolve(c);
 	for (hipEvent_t e : c->ev_pool)
 		HIPCHECK(hipEventDestroy(e));
-	HIPCHECK(hipStreamDestroy(c->stream));
 	delete c;
 }
 
