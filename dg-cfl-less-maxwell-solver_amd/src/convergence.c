@@ -12,6 +12,8 @@
 #include <simulation/simulation.h>
 #include <utils/convergence.h>
 
+#include <io/io_hdf5.h>
+
 #include "host_internal.h"
 
 static gdn_real order_of(gdn_real e1, gdn_real e0, gdn_real d1, gdn_real d0)
@@ -21,15 +23,12 @@ static gdn_real order_of(gdn_real e1, gdn_real e0, gdn_real d1, gdn_real d0)
 	return log(e1 / e0) / log(d1 / d0);
 }
 
-static void dump_raw(const gdn_simulation *simu, int run)
+/* the field dump of src/utils/convergence.c:88-95 (XDMF + raw binary) */
+static void dump_fields(gdn_simulation *simu, int run)
 {
-	char name[64];
-	snprintf(name, sizeof(name), "wave_run_%03d.bin", run);
-	FILE *f = fopen(name, "wb");
-	if (!f)
-		return;
-	fwrite(simu->wn, sizeof(gdn_real), (size_t)simu->wlen, f);
-	fclose(f);
+	char name[1024];
+	snprintf(name, sizeof(name), "wave_run_%03d", run);
+	io_save_all(simu, name, simu->itermax);
 }
 
 void utils_l2_time_convergence(gdn_model *model, const int nb_run,
@@ -64,7 +63,7 @@ void utils_l2_time_convergence(gdn_model *model, const int nb_run,
 			k > 0 ? order_of(err[k], err[k - 1], dt[k], dt[k - 1]) : -1;
 		simulation_dump_info(k, &simu, err[k], ord);
 		if (export_xdmf)
-			dump_raw(&simu, k + 1);
+			dump_fields(&simu, k + 1);
 	}
 	simulation_free(&simu);
 	free(err);
