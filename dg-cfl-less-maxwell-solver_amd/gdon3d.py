@@ -161,6 +161,20 @@ def load_library(path=LIB_PATH):
         "dg_batch_times": (None, [C.c_void_p, c_double_p]),
         "dg_batch_finish": (None, [C.c_void_p]),
         "dg_batch_free": (None, [C.c_void_p]),
+        "dg_stepper_device_context": (C.c_void_p, [C.c_void_p]),
+        "gdh_mlp_nparams": (C.c_int, []),
+        "gdh_mlp_create": (C.c_void_p, [C.c_uint64]),
+        "gdh_mlp_destroy": (None, [C.c_void_p]),
+        "gdh_mlp_get_params": (None, [C.c_void_p, c_float_p]),
+        "gdh_mlp_set_params": (None, [C.c_void_p, c_float_p]),
+        "gdh_mlp_set_normalization": (None, [C.c_void_p, c_float_p]),
+        "gdh_mlp_grad": (C.c_double, [C.c_void_p, c_float_p, c_float_p, C.c_int, c_float_p]),
+        "gdh_mlp_train": (C.c_double, [C.c_void_p, c_float_p, c_float_p, C.c_int, C.c_int,
+                                       C.c_int, C.c_float, C.c_uint64, c_double_p]),
+        "gdh_mlp_predict": (None, [C.c_void_p, c_float_p, C.c_int, c_float_p]),
+        "gdh_learned_data": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_float_p, c_float_p]),
+        "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                           C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
         "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
         "t10_get_inv_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
@@ -179,6 +193,19 @@ def fnptr(name):
     """Address of an exported C function, for a gdn_model callback field."""
     lib = load_library()
     return C.cast(getattr(lib, name), C.c_void_p).value
+
+
+c_float_p = C.POINTER(C.c_float)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_float_p)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def dptr(a):
@@ -388,3 +415,78 @@ class Batch:
         if self.h:
             self.lib.dg_batch_free(self.h)
             self.h = None
+
+
+class BlockNet:
+    """The learned block preconditioner's network (gdh_mlp_*)."""
+
+    def __init__(self, seed=1):
+        self.lib = load_library()
+        self.h = self.lib.gdh_mlp_create(seed)
+        self.n = self.lib.gdh_mlp_nparams()
+
+    def params(self):
+        out = np.zeros(self.n, np.float32)
+        self.lib.gdh_mlp_get_params(self.h, fptr(out))
+        return out
+
+    def set_params(self, p):
+        self.lib.gdh_mlp_set_params(self.h, fptr(np.ascontiguousarray(p, np.float32)))
+
+    def set_normalization(self, norm):
+        self.norm = np.ascontiguousarray(norm, np.float32)
+        self.lib.gdh_mlp_set_normalization(self.h, fptr(self.norm))
+
+    def grad(self, x, y):
+        g = np.zeros(self.n, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        loss = self.lib.gdh_mlp_grad(self.h, fptr(x), fptr(y), x.shape[0], fptr(g))
+        return loss, g
+
+    def train(self, x, y, steps, batch=1024, lr=1e-3, seed=7):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        hist = np.zeros(steps)
+        self.lib.gdh_mlp_train(self.h, fptr(x), fptr(y), x.shape[0], steps, batch, lr, seed,
+                               dptr(hist))
+        return hist
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], 100), np.float32)
+        self.lib.gdh_mlp_predict(self.h, fptr(x), x.shape[0], fptr(y))
+        return y
+
+    def free(self):
+        if self.h:
+            self.lib.gdh_mlp_destroy(self.h)
+            self.h = None
+
+
+def learned_data(stepper, idx):
+    """Features x[n][7] and targets D^-1 - I [n][100] of sweep items idx."""
+    lib = load_library()
+    idx = np.ascontiguousarray(idx, np.int32)
+    x = np.zeros((len(idx), 7), np.float32)
+    y = np.zeros((len(idx), 100), np.float32)
+    ctx = lib.dg_stepper_device_context(stepper.h)
+    lib.gdh_learned_data(ctx, len(idx), iptr(idx), fptr(x), fptr(y))
+    return x, y
+
+
+def learned_solve_stats(stepper, net, idx, tol=1e-8, maxit=50):
+    """(mean iterations, max, fallback rate, max rel. L2 to exact, solves)"""
+    lib = load_library()
+    idx = np.ascontiguousarray(idx, np.int32)
+    out = np.zeros(5)
+    ctx = lib.dg_stepper_device_context(stepper.h)
+    lib.gdh_learned_solve_stats(ctx, net.h if net is not None else None, len(idx), iptr(idx),
+                                tol, maxit, dptr(out), None)
+    return out
+
+
+def normalization(x, y):
+    """mu_x, sd_x, mu_y, sd_y of training data (sd floored)."""
+    return np.concatenate([x.mean(0), np.maximum(x.std(0), 1e-6),
+                           y.mean(0), np.maximum(y.std(0), 1e-6)]).astype(np.float32)

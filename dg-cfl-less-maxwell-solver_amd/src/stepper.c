@@ -159,3 +159,8 @@ void dg_stepper_transport_stats(const gdn_stepper *st, int out[3])
 		out[0] = out[1] = out[2] = 0;
 	}
 }
+
+void *dg_stepper_device_context(gdn_stepper *st)
+{
+	return st->ctx;
+}

@@ -56,6 +56,10 @@ int dg_stepper_levels(const gdn_stepper *st);
  * number of sweeps in one step (gdh_transport_stats). */
 void dg_stepper_transport_stats(const gdn_stepper *st, int out[3]);
 
+/* The stepper's device context (a gdh_ctx *, include/gdn_hip.h), for the
+ * gdh_* calls that work on a set-up context (e.g. gdh_learned_data). */
+void *dg_stepper_device_context(gdn_stepper *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -193,6 +193,44 @@ void gdh_batch_times(gdh_ctx *ctx, double *t);
 void gdh_batch_ghosts(gdh_ctx *ctx);
 void gdh_batch_error_l2(gdh_ctx *ctx, double *err);
 
+/* ---- learned block preconditioner of the transport solve (no counterpart
+ * in the reference, whose KLU factors each block exactly). A network
+ * 7 -> 64 -> 64 -> 100 (tanh; bf16 MFMA products, fp32 accumulation,
+ * fp32 master weights) maps an item's 7 block parameters (theta dt/det times
+ * v . gamma_k, k < 3, and times the outflow coefficients s_f, f < 4) to
+ * P - I with P ~ D^-1. Parameters are a flat fp32 vector of
+ * gdh_mlp_nparams() values: W1[64][16] (7 inputs used), b1[64], W2[64][64],
+ * b2[64], W3[112][64] (100 rows used), b3[112]. */
+typedef struct gdh_mlp gdh_mlp;
+int gdh_mlp_nparams(void);
+/* Xavier-uniform weights from SplitMix64(seed), zero biases */
+gdh_mlp *gdh_mlp_create(uint64_t seed);
+void gdh_mlp_destroy(gdh_mlp *mlp);
+void gdh_mlp_get_params(gdh_mlp *mlp, float *out);
+void gdh_mlp_set_params(gdh_mlp *mlp, const float *in);
+/* mu_x[7], sd_x[7], mu_y[100], sd_y[100]: inputs and targets are used as
+ * (x - mu) / sd */
+void gdh_mlp_set_normalization(gdh_mlp *mlp, const float *norm214);
+/* gradient (grad[nparams]) of the mean squared normalised error over the
+ * n items x[n][7], y[n][100]; returns that loss */
+double gdh_mlp_grad(gdh_mlp *mlp, const float *x, const float *y, int n, float *grad);
+/* Adam (lr; beta 0.9, 0.999) on `steps` minibatches of `batch` items drawn
+ * with SplitMix64(seed); loss_hist[steps] or NULL; returns the last loss */
+double gdh_mlp_train(gdh_mlp *mlp, const float *x, const float *y, int n, int steps,
+		     int batch, float lr, uint64_t seed, double *loss_hist);
+/* y[n][100] = prediction of D^-1 - I (denormalised) */
+void gdh_mlp_predict(gdh_mlp *mlp, const float *x, int n, float *y);
+/* training data of a kinetic context after gdh_relax_setup: x[n][7] and
+ * y[n][100] = (stored exact D^-1) - I of the sweep items idx[n] */
+void gdh_learned_data(gdh_ctx *ctx, int n, const int *idx, float *x, float *y);
+/* block solves D u = r (6 random sides per item) by Richardson with P from
+ * mlp (identity if NULL) to |r - Du| <= tol |r|, exact D^-1 r after maxit
+ * iterations; out[5] = mean iterations (converged), largest, fallback rate,
+ * largest relative L2 to the exact solution, number of solves; iters[6 n]
+ * (or NULL) per solve, -1 = fallback */
+void gdh_learned_solve_stats(gdh_ctx *ctx, gdh_mlp *mlp, int n, const int *idx,
+			     double tol, int maxit, double *out, int *iters);
+
 /* Copies `bytes` (read) into a second buffer (written) `reps` times with a
  * plain streaming kernel (k_copy_calib) after one warm-up launch; returns
  * the mean milliseconds per launch. No counterpart in the reference: the
