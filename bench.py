@@ -355,8 +355,8 @@ def main():
                          "1 GiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the design for sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the solver has no multi-GPU path "
+              "(sharding by velocity or by element partition is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     lib = G.load_library()

@@ -7,7 +7,7 @@
  * P = I + NN(x) ~ D^-1. The block solve D u = r then runs as preconditioned
  * Richardson u <- u + P (r - D u) to a relative residual of 1e-8, with the
  * exact inverse (the stored Gauss-Jordan factor) as the fallback and as the
- * judge of the result.
+ * yardstick of every result.
  *
  * Network: 7 -> 64 -> 64 -> 100, tanh, inputs and outputs normalised by
  * training-set means and deviations. Products on MFMA in bf16 with fp32
