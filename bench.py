@@ -399,7 +399,8 @@ def main():
     default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
                         and args.precision == "fp64")
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
-          f"levels={st.levels} setup={t_setup:.2f}s", file=sys.stderr, flush=True)
+          f"levels={st.levels} sweep_grid={G.sweep_grid(st)} setup={t_setup:.2f}s",
+          file=sys.stderr, flush=True)
 
     st.advance(args.warmup)
     st.sync()

@@ -60,7 +60,8 @@ struct gdh_ctx {
 	int4 *nbr, *nbmap;
 	double *xnode, *xghost;
 	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
-	 * GH[q][6 bf + a]; kinetic: w = reference layout, ghosts included.
+	 * GH[q][6 bf + a]; kinetic: w = item-major state (kin_at), ghosts
+	 * included.
 	 * Elements are double, or float with fp32. */
 	void *w[2];
 	int cur;
@@ -116,6 +117,7 @@ struct gdh_ctx {
 		return m;
 	}
 	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
+	int sweep_poll; /* 1: waiting items poll whole traces */
 
 	/* optional event timing of the main kernels: pairs of events recorded
 	 * on the stream around each kernel group, resolved when read */
@@ -203,6 +205,24 @@ __device__ inline T *macro_ptr(const StateRef<T> &s, long ipg, int q)
 	if (ipg < 10 * s.ne)
 		return s.w + (q * 10 + ipg % 10) * s.ne + s.pos[ipg / 10];
 	return s.gh + q * s.ngh + (ipg - 10 * s.ne);
+}
+
+/*
+ * Kinetic state on the device, item-major: the 60 values of item (v, e)
+ * (6 fields x 10 nodes) are contiguous, K[(v ne + e) 60 + 10 iw + i], and
+ * the ghost values follow, K[60 nv ne + (v nb + b) 36 + 6 iw + a]. The
+ * transport sweep reads and writes whole items and face traces of one
+ * element, which this layout keeps within 480 contiguous bytes. The
+ * reference layout f[(iw nv + v) neq + ipg] (src/simulation/simulation.c:22-40,
+ * src/dg/relaxation.c:106) is restored at host transfers. neq = 10 ne + 6 nb.
+ */
+__host__ __device__ inline long kin_at(long ne, long neq, int nv, long ipg, int iw, int v)
+{
+	const long ne10 = 10 * ne;
+	if (ipg < ne10)
+		return ((long)v * ne + ipg / 10) * 60 + 10 * iw + ipg % 10;
+	const long g = ipg - ne10, a = g % 6;
+	return 60L * nv * ne + 6L * v * (neq - ne10) + 6 * (g - a) + 6 * iw + a;
 }
 
 /* launchers implemented in rk3.hip and sweep.hip */

@@ -8,7 +8,8 @@
  *    nbr[e] (int4: neighbour or -1-bf), nbmap[e] (int4: 4 bits per face node);
  *  - RK3 state in the reference AoS layout w[ipg * 6 + iw] (each element's
  *    60 values are one contiguous 480 B block), ping-pong buffers + k2;
- *  - kinetic state in the reference layout f[(iw * nv + iv) * neq + ipg].
+ *  - kinetic state item-major (kin_at in ctx.h), the reference layout
+ *    f[(iw * nv + iv) * neq + ipg] restored at host transfers.
  */
 #include "ctx.h"
 
@@ -146,7 +147,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 				   hipMemcpyHostToDevice));
 
 	const size_t wlen = (size_t)c->m * c->neq;
-	/* RK3: interior SoA buffers; kinetic: full reference-layout buffers */
+	/* RK3: interior SoA buffers; kinetic: full item-major buffers */
 	const size_t blen = scheme ? wlen : 60 * ne;
 	for (int b = 0; b < 2; b++) {
 		c->w[b] = dmalloc(c->rsz * blen);
@@ -202,7 +203,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 				   hipMemcpyHostToDevice));
 		c->stage = (double *)dmalloc(sizeof(double) * wlen);
 	}
-	if (scheme && c->fp32)
+	if (scheme)
 		c->stage = (double *)dmalloc(sizeof(double) * wlen);
 	c->hostvals = nullptr;
 	c->hostvals_len = 0;
@@ -285,19 +286,28 @@ __global__ void k_soa_to_aos(double *__restrict__ a, StateRef<T> s, long n)
 	a[k] = (double)*macro_ptr(s, k / 6, (int)(k % 6));
 }
 
-/* kinetic state in the reference layout, binary64 <-> binary32 */
-__global__ void k_narrow(const double *__restrict__ a, float *__restrict__ b, long n)
+/* kinetic state: reference layout a[(iw nv + v) neq + ipg] (binary64) <->
+ * item-major device state (kin_at), T = double or float */
+template <typename T>
+__global__ void k_ref_to_kin(const double *__restrict__ a, T *__restrict__ b, long ne,
+			     long neq, int nv, long n)
 {
 	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k < n)
-		b[k] = (float)a[k];
+	if (k >= n)
+		return;
+	const long row = k / neq, ipg = k - row * neq;
+	b[kin_at(ne, neq, nv, ipg, (int)(row / nv), (int)(row % nv))] = (T)a[k];
 }
 
-__global__ void k_widen(const float *__restrict__ b, double *__restrict__ a, long n)
+template <typename T>
+__global__ void k_kin_to_ref(const T *__restrict__ b, double *__restrict__ a, long ne,
+			     long neq, int nv, long n)
 {
 	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k < n)
-		a[k] = (double)b[k];
+	if (k >= n)
+		return;
+	const long row = k / neq, ipg = k - row * neq;
+	a[k] = (double)b[kin_at(ne, neq, nv, ipg, (int)(row / nv), (int)(row % nv))];
 }
 
 template <typename T> static StateRef<T> state_ref(gdh_ctx *c)
@@ -314,46 +324,44 @@ template <typename T> static StateRef<T> state_ref(gdh_ctx *c)
 extern "C" void gdh_upload_state(gdh_ctx *c, const double *wn)
 {
 	const size_t len = (size_t)c->m * c->neq;
-	if (c->scheme && !c->fp32) {
-		HIPCHECK(hipMemcpyAsync(c->w[c->cur], wn, sizeof(double) * len,
-					hipMemcpyHostToDevice, c->stream));
-	} else {
-		HIPCHECK(hipMemcpyAsync(c->stage, wn, sizeof(double) * len,
-					hipMemcpyHostToDevice, c->stream));
-		if (c->scheme)
-			k_narrow<<<nblocks(len, 256), 256, 0, c->stream>>>(
-				c->stage, (float *)c->w[c->cur], (long)len);
-		else if (c->fp32)
-			k_aos_to_soa<float><<<nblocks(len, 256), 256, 0, c->stream>>>(
-				c->stage, state_ref<float>(c), (long)len);
-		else
-			k_aos_to_soa<double><<<nblocks(len, 256), 256, 0, c->stream>>>(
-				c->stage, state_ref<double>(c), (long)len);
-		HIPCHECK(hipGetLastError());
-	}
+	HIPCHECK(hipMemcpyAsync(c->stage, wn, sizeof(double) * len, hipMemcpyHostToDevice,
+				c->stream));
+	const int nb = nblocks(len, 256);
+	if (c->scheme && c->fp32)
+		k_ref_to_kin<float><<<nb, 256, 0, c->stream>>>(
+			c->stage, (float *)c->w[c->cur], c->ne, c->neq, c->kp.nv, (long)len);
+	else if (c->scheme)
+		k_ref_to_kin<double><<<nb, 256, 0, c->stream>>>(
+			c->stage, (double *)c->w[c->cur], c->ne, c->neq, c->kp.nv, (long)len);
+	else if (c->fp32)
+		k_aos_to_soa<float><<<nb, 256, 0, c->stream>>>(c->stage, state_ref<float>(c),
+							       (long)len);
+	else
+		k_aos_to_soa<double><<<nb, 256, 0, c->stream>>>(c->stage, state_ref<double>(c),
+								(long)len);
+	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
 extern "C" void gdh_download_state(gdh_ctx *c, double *wn)
 {
 	const size_t len = (size_t)c->m * c->neq;
-	if (c->scheme && !c->fp32) {
-		HIPCHECK(hipMemcpyAsync(wn, c->w[c->cur], sizeof(double) * len,
-					hipMemcpyDeviceToHost, c->stream));
-	} else {
-		if (c->scheme)
-			k_widen<<<nblocks(len, 256), 256, 0, c->stream>>>(
-				(const float *)c->w[c->cur], c->stage, (long)len);
-		else if (c->fp32)
-			k_soa_to_aos<float><<<nblocks(len, 256), 256, 0, c->stream>>>(
-				c->stage, state_ref<float>(c), (long)len);
-		else
-			k_soa_to_aos<double><<<nblocks(len, 256), 256, 0, c->stream>>>(
-				c->stage, state_ref<double>(c), (long)len);
-		HIPCHECK(hipGetLastError());
-		HIPCHECK(hipMemcpyAsync(wn, c->stage, sizeof(double) * len,
-					hipMemcpyDeviceToHost, c->stream));
-	}
+	const int nb = nblocks(len, 256);
+	if (c->scheme && c->fp32)
+		k_kin_to_ref<float><<<nb, 256, 0, c->stream>>>(
+			(const float *)c->w[c->cur], c->stage, c->ne, c->neq, c->kp.nv, (long)len);
+	else if (c->scheme)
+		k_kin_to_ref<double><<<nb, 256, 0, c->stream>>>(
+			(const double *)c->w[c->cur], c->stage, c->ne, c->neq, c->kp.nv, (long)len);
+	else if (c->fp32)
+		k_soa_to_aos<float><<<nb, 256, 0, c->stream>>>(c->stage, state_ref<float>(c),
+							       (long)len);
+	else
+		k_soa_to_aos<double><<<nb, 256, 0, c->stream>>>(c->stage, state_ref<double>(c),
+								(long)len);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(wn, c->stage, sizeof(double) * len, hipMemcpyDeviceToHost,
+				c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -423,7 +431,7 @@ static const double *stage_hostvals(gdh_ctx *c, const double *h, size_t n)
 /* ===================================================================== */
 
 /* Write the macro state wv (6 values) of DoF ipg into the state, as w
- * itself (RK3, SoA) or as f_eq(w) (kinetic, reference layout). */
+ * itself (RK3, SoA) or as f_eq(w) (kinetic, item-major). */
 template <typename T>
 __device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 				 const KinPar &kp, int neq, long ipg,
@@ -439,7 +447,7 @@ __device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 	maxwell_qM(wv, qm);
 	for (int iw = 0; iw < 6; iw++)
 		for (int v = 0; v < kp.nv; v++)
-			kin[(long)(iw * kp.nv + v) * neq + ipg] =
+			kin[kin_at(s.ne, neq, kp.nv, ipg, iw, v)] =
 				(T)feq_comp(kp, wv, qm, iw, v);
 }
 
@@ -570,7 +578,7 @@ __global__ __launch_bounds__(ERR_BS) void k_err_partial(
 				for (int q = 0; q < 6; q++) {
 					double s = 0.0;
 					for (int v = 0; v < kp.nv; v++)
-						s += (double)st[(long)(q * kp.nv + v) * neq + ipg];
+						s += (double)st[kin_at(sr.ne, neq, kp.nv, ipg, q, v)];
 					wl[q] = s;
 				}
 			}

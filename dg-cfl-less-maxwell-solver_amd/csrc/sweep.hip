@@ -233,13 +233,28 @@ struct SweepTables {
 	double ML[4][10][6];
 };
 
-/* One prepared item: everything that does not depend on the inflow
- * neighbours' new values. Slot s < nslot is the s-th interior inflow face. */
+/*
+ * One prepared item: everything that does not depend on the new values of
+ * its inflow neighbours. Slot s < nslot is the s-th interior, not lagged,
+ * inflow face: its face index, coefficient c_f, neighbour and node map.
+ */
 struct ItemBuf {
 	double base[6][10];
-	double G[3][10][6]; /* theta G_f of the slot's face */
-	int v, e, nslot, pad;
-	int nb[3], map[3];
+	double D[10][10]; /* D^-1 */
+	double cs[3];
+	int v, e, nslot;
+	int fsl[3], nb[3], map[3];
+};
+
+/* The global loads of one item's preparation, issued ahead of their use
+ * and kept as loaded (no arithmetic on them until prep_compute, so that
+ * none of them waits for another): lane t < 60 its own value (iw, i) =
+ * (t / 10, t % 10), D^-1 entries t and 64 + t, and for t < 36 the trace
+ * value (field t / 6, face node t % 6) of each ghost or lagged inflow
+ * face. */
+template <typename R> struct PrepLoads {
+	R fv, u[4];
+	double d0, d1;
 };
 
 /*
@@ -250,102 +265,128 @@ struct ItemBuf {
  *   sol = D^-1 [ f - dt/det Minv ((1-theta) F_ee f + sum_f s_f L_f S u_f) ]
  *       = D^-1 (2 f - sum_f c_f ML_f u_f) - f,   c_f = s_f dt/det,
  * since (dt/det) Minv (1-theta) F_ee = D - I for theta = 1/2, with
- * ML_f = Minv L_f S. prepare() forms base = D^-1 (2 f - sum_f c_f ML_f u_f^old)
- * - f with u^old = (1-theta) f_nb or the ghost value, and
- * theta G_f = theta c_f D^-1 ML_f for the interior inflow faces; after the
- * wait only sum_f theta G_f sol_nb remains. One round of global loads, then
- * LDS only. Lanes: (iw, i) = (field, node) for vectors, (r, b) = (row, face
- * node) for the 10x6 matrices G_f.
+ * ML_f = Minv L_f S. Every item publishes w = (1-theta) f + theta sol, which
+ * is u_f for its downstream neighbours. Before the wait: base = D^-1 (2 f -
+ * sum_g c_g ML_g u_g) - f over the ghost and lagged inflow faces g
+ * (prep_issue: the loads; prep_compute: LDS only). After it:
+ *   sol = base - D^-1 sum_s c_s ML_s w_nb,s
+ * over the interior inflow slots s (sweep_finish). Lanes: t = 10 iw + i
+ * (field, node) for vectors, t = 6 q + b (field, face node) for traces.
  */
+/* state index of the inflow trace value of lane t < 36 on face f */
+__device__ inline long trace_index(const ItemDesc &d, int f, int t, int ne, int neq, int nv)
+{
+	if (d.flags & (16 << f)) /* ghost values of boundary face nb[f] */
+		return 60L * nv * ne + 6L * d.v * (neq - 10L * ne) + 36L * d.nb[f] + t;
+	const int q = t / 6, b = t % 6;
+	return ((long)d.v * ne + d.nb[f]) * 60 + 10 * q + ((d.map[f] >> (4 * b)) & 15);
+}
+
+/* inflow faces whose trace is known before the wait: ghost or lagged */
+__device__ inline int given_faces(const ItemDesc &d)
+{
+	return d.flags & ((d.flags >> 4) | (d.flags >> 8)) & 15;
+}
+
+/* interior inflow faces, not lagged: the slots */
+__device__ inline int slot_faces(const ItemDesc &d)
+{
+	return d.flags & ~(d.flags >> 4) & ~(d.flags >> 8) & 15;
+}
+
 template <typename R>
-__device__ inline void sweep_prepare(int it, const ItemDesc &d,
-				     const R *__restrict__ fo, const R *__restrict__ xl,
-				     const double *__restrict__ dinv, int ne,
-				     int neq, int nv, const SweepTables &T,
-				     ItemBuf &B, double (&Z)[6][10])
+__device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict__ fo,
+				  const double *__restrict__ dinv, int ne, int neq, int nv,
+				  PrepLoads<R> &P)
 {
 	const int t = threadIdx.x;
-	const int iw = t / 10, i = t % 10;
-	const int r = t / 6, b6 = t % 6;
-	const int v = d.v, e = d.e;
+	P.fv = t < 60 ? fo[((long)d.v * ne + d.e) * 60 + t] : (R)0;
 	const double *D = dinv + 100L * it;
+	P.d0 = D[t];
+	P.d1 = t < 36 ? D[64 + t] : 0.0;
+	const int gf = given_faces(d);
+#pragma unroll
+	for (int f = 0; f < 4; f++)
+		P.u[f] = (gf & (1 << f)) && t < 36 ? fo[trace_index(d, f, t, ne, neq, nv)] : (R)0;
+}
 
-	double fv = 0.0, Dr[10], Dg[10];
-	if (t < 60) {
-		const long bq = (long)(iw * nv + v) * neq;
-		fv = (double)fo[bq + 10L * e + i];
-		double y = 0.0;
+/* S: per-block scratch of 108 values (the given traces U[3][36], then Z).
+ * u = the ghost value, or on a lagged face (1 - theta) f_nb + theta x_nb
+ * with x_nb from xl (none when xl is null: both halves of a lagged trace
+ * are given values). */
+template <typename R>
+__device__ inline void prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
+				    const R *__restrict__ xl, int ne, int neq, int nv,
+				    const SweepTables &T, ItemBuf &B, double (&S)[3][36])
+{
+	const int t = threadIdx.x;
+	const int gf = given_faces(d);
+	(&B.D[0][0])[t] = P.d0;
+	if (t < 36)
+		(&B.D[0][0])[64 + t] = P.d1;
+	{
+		int j = 0;
 #pragma unroll
 		for (int f = 0; f < 4; f++) {
-			if (!(d.flags & (1 << f)))
+			if (!(gf & (1 << f)))
 				continue;
-			double u[6];
-			if (d.flags & (16 << f)) {
-#pragma unroll
-				for (int b = 0; b < 6; b++)
-					u[b] = (double)fo[bq + 10L * ne + 6L * d.nb[f] + b];
-			} else if (d.flags & (256 << f)) {
-				/* lagged: both halves of the trace known (the new
-				 * values from xl; none when xl is null) */
-#pragma unroll
-				for (int b = 0; b < 6; b++) {
-					const long g = bq + 10L * d.nb[f] + ((d.map[f] >> (4 * b)) & 15);
-					u[b] = (1.0 - THETA) * (double)fo[g] +
-					       (xl ? THETA * (double)xl[g] : 0.0);
+			if (t < 36) {
+				double u = (double)P.u[f];
+				if (!(d.flags & (16 << f))) {
+					u = (1.0 - THETA) * u;
+					if (xl)
+						u += THETA * (double)xl[trace_index(d, f, t, ne,
+										    neq, nv)];
 				}
-			} else {
-#pragma unroll
-				for (int b = 0; b < 6; b++)
-					u[b] = (1.0 - THETA) *
-					       (double)fo[bq + 10L * d.nb[f] +
-							  ((d.map[f] >> (4 * b)) & 15)];
+				S[j][t] = u;
 			}
-			double s = 0.0;
-#pragma unroll
-			for (int b = 0; b < 6; b++)
-				s += T.ML[f][i][b] * u[b];
-			y += d.cf[f] * s;
-		}
-		Z[iw][i] = 2.0 * fv - y;
-#pragma unroll
-		for (int l = 0; l < 10; l++) {
-			Dr[l] = D[10 * i + l];
-			Dg[l] = D[10 * r + l];
-		}
-		/* theta G_f = theta c_f D^-1 ML_f on the interior inflow faces */
-		int slot = 0;
-#pragma unroll
-		for (int f = 0; f < 4; f++) {
-			if ((d.flags & ~(d.flags >> 4) & ~(d.flags >> 8) & (1 << f)) == 0)
-				continue;
-			double s = 0.0;
-#pragma unroll
-			for (int l = 0; l < 10; l++)
-				s += Dg[l] * T.ML[f][l][b6];
-			B.G[slot][r][b6] = THETA * (d.cf[f] * s);
-			slot++;
+			j++;
 		}
 	}
 	if (t == 0) {
-		int slot = 0;
-		B.v = v;
-		B.e = e;
+		const int sf = slot_faces(d);
+		int s = 0;
+		B.v = d.v;
+		B.e = d.e;
 		for (int f = 0; f < 4; f++) {
-			if ((d.flags & ~(d.flags >> 4) & ~(d.flags >> 8) & (1 << f)) == 0)
+			if (!(sf & (1 << f)))
 				continue;
-			B.nb[slot] = d.nb[f];
-			B.map[slot] = d.map[f];
-			slot++;
+			B.fsl[s] = f;
+			B.cs[s] = d.cf[f];
+			B.nb[s] = d.nb[f];
+			B.map[s] = d.map[f];
+			s++;
 		}
-		B.nslot = slot;
+		B.nslot = s;
 	}
+	__syncthreads();
+	const int iw = t / 10, i = t % 10;
+	double y = 0.0;
+	if (t < 60) {
+		int j = 0;
+#pragma unroll
+		for (int f = 0; f < 4; f++) {
+			if (!(gf & (1 << f)))
+				continue;
+			double s = 0.0;
+#pragma unroll
+			for (int b = 0; b < 6; b++)
+				s += T.ML[f][i][b] * S[j][6 * iw + b];
+			y += d.cf[f] * s;
+			j++;
+		}
+	}
+	__syncthreads(); /* traces read; S now holds Z */
+	double(*Z)[10] = reinterpret_cast<double(*)[10]>(&S[0][0]);
+	if (t < 60)
+		Z[iw][i] = 2.0 * (double)P.fv - y;
 	__syncthreads();
 	if (t < 60) {
 		double s = 0.0;
 #pragma unroll
 		for (int l = 0; l < 10; l++)
-			s += Dr[l] * Z[iw][l];
-		B.base[iw][i] = s - fv;
+			s += B.D[i][l] * Z[iw][l];
+		B.base[iw][i] = s - (double)P.fv;
 	}
 	__syncthreads();
 }
@@ -360,43 +401,58 @@ __device__ inline void ll_store(u64 *p, u64 x)
 	__hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Hand-off words of one value: binary64 as two words (epoch << 32 | high
- * half), (epoch << 32 | low half); binary32 as one word (epoch << 32 |
- * bits). */
+/* Hand-off words of the 60 values of one item (v, e) at ll + 60 words (v ne
+ * + e): binary64 as a block of 60 high-half words (epoch << 32 | high
+ * half) followed by 60 low-half words (epoch << 32 | low half), so that
+ * each store instruction of an item writes whole lines; binary32 as 60
+ * words (epoch << 32 | bits). */
 template <typename R> struct Handoff;
 template <> struct Handoff<double> {
 	static constexpr int words = 2;
-	__device__ static void put(u64 *p, double x, unsigned epoch)
+	__device__ static void put(u64 *item, int k, double x, unsigned epoch)
 	{
 		const u64 bits = (u64)__double_as_longlong(x), tag = (u64)epoch << 32;
-		ll_store(p, tag | (bits >> 32));
-		ll_store(p + 1, tag | (bits & 0xffffffffull));
+		ll_store(item + k, tag | (bits >> 32));
+		ll_store(item + 60 + k, tag | (bits & 0xffffffffull));
+	}
+	__device__ static void load(const u64 *item, int k, u64 (&w)[2])
+	{
+		w[0] = ll_load(item + k);
+		w[1] = ll_load(item + 60 + k);
 	}
 	/* true (and x set) when both words carry the epoch */
-	__device__ static bool get(const u64 *p, unsigned epoch, double &x)
+	__device__ static bool decode(const u64 (&w)[2], unsigned epoch, double &x)
 	{
-		const u64 hi = ll_load(p), lo = ll_load(p + 1);
-		if ((unsigned)(hi >> 32) != epoch || (unsigned)(lo >> 32) != epoch)
+		if ((unsigned)(w[0] >> 32) != epoch || (unsigned)(w[1] >> 32) != epoch)
 			return false;
-		x = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+		x = __longlong_as_double((long long)((w[0] << 32) | (w[1] & 0xffffffffull)));
 		return true;
 	}
 };
 template <> struct Handoff<float> {
 	static constexpr int words = 1;
-	__device__ static void put(u64 *p, float x, unsigned epoch)
+	__device__ static void put(u64 *item, int k, float x, unsigned epoch)
 	{
-		ll_store(p, ((u64)epoch << 32) | (u64)__float_as_uint(x));
+		ll_store(item + k, ((u64)epoch << 32) | (u64)__float_as_uint(x));
 	}
-	__device__ static bool get(const u64 *p, unsigned epoch, double &x)
+	__device__ static void load(const u64 *item, int k, u64 (&w)[2])
 	{
-		const u64 w = ll_load(p);
-		if ((unsigned)(w >> 32) != epoch)
+		w[0] = ll_load(item + k);
+	}
+	__device__ static bool decode(const u64 (&w)[2], unsigned epoch, double &x)
+	{
+		if ((unsigned)(w[0] >> 32) != epoch)
 			return false;
-		x = (double)__uint_as_float((unsigned)(w & 0xffffffffull));
+		x = (double)__uint_as_float((unsigned)(w[0] & 0xffffffffull));
 		return true;
 	}
 };
+
+/* hand-off words of item (v, e) */
+template <typename R> __device__ inline u64 *ll_item(u64 *ll, int v, int e, int ne)
+{
+	return ll + Handoff<R>::words * 60 * ((long)v * ne + e);
+}
 
 /*
  * Lane s (< nslot) checks the first word of one value of slot s's
@@ -405,41 +461,40 @@ template <> struct Handoff<float> {
  * not flood the memory side with coherent loads.
  */
 template <typename R>
-__device__ inline bool ll_sentinels(const ItemBuf &B, const u64 *ll, int ne,
-				    unsigned epoch)
+__device__ inline bool ll_sentinels(const ItemBuf &B, u64 *ll, int ne, unsigned epoch)
 {
 	const int t = threadIdx.x;
 	bool wait = false;
-	if (t < B.nslot) {
-		const u64 *p = ll + Handoff<R>::words *
-					    (((long)B.v * ne + B.nb[t]) * 60 + (B.map[t] & 15));
-		wait = (unsigned)(ll_load(p) >> 32) != epoch;
-	}
+	if (t < B.nslot)
+		wait = (unsigned)(ll_load(ll_item<R>(ll, B.v, B.nb[t], ne) + (B.map[t] & 15)) >>
+				  32) != epoch;
 	return __ballot(wait) == 0;
 }
 
 /*
- * One pass over the inflow values that lane (q, b) (t < 36) still misses
- * (bit s of pending): each complete one goes to X[s][q][b]. Returns the
- * slots still pending.
+ * One pass over the inflow values that lane t = 6 q + b (t < 36) still
+ * misses (bit s of pending): each complete one goes to X[s][t]. Returns the
+ * slots still pending. All loads are issued before any is decoded.
  */
 template <typename R>
-__device__ inline unsigned ll_poll(const ItemBuf &B, const u64 *ll, int ne,
-				   unsigned epoch, unsigned pending,
-				   double (&X)[3][6][6])
+__device__ inline unsigned ll_poll(const ItemBuf &B, u64 *ll, int ne, unsigned epoch,
+				   unsigned pending, double (&X)[3][36])
 {
 	const int t = threadIdx.x;
 	const int q = t / 6, b = t % 6;
+	u64 w[3][2];
 #pragma unroll
 	for (int s = 0; s < 3; s++) {
-		if (!(pending & (1u << s)))
-			continue;
-		const int j = (B.map[s] >> (4 * b)) & 15;
-		const u64 *p = ll + Handoff<R>::words *
-					    (((long)B.v * ne + B.nb[s]) * 60 + q * 10 + j);
+		w[s][0] = w[s][1] = 0;
+		if (pending & (1u << s))
+			Handoff<R>::load(ll_item<R>(ll, B.v, B.nb[s], ne),
+					 q * 10 + ((B.map[s] >> (4 * b)) & 15), w[s]);
+	}
+#pragma unroll
+	for (int s = 0; s < 3; s++) {
 		double x;
-		if (Handoff<R>::get(p, epoch, x)) {
-			X[s][q][b] = x;
+		if ((pending & (1u << s)) && Handoff<R>::decode(w[s], epoch, x)) {
+			X[s][t] = x;
 			pending &= ~(1u << s);
 		}
 	}
@@ -461,39 +516,40 @@ __device__ inline bool spin_expired(int spins, int *status)
 	return false;
 }
 
-/* sol = base - sum_s theta G_s sol_nb; published to fn (and ll), rounded
- * once to R */
-template <bool PERSIST, typename R>
-__device__ inline void sweep_finish(const ItemBuf &B, const double (*X)[6][6],
-				    R *__restrict__ fn, u64 *ll, int nv, int ne,
-				    int neq, unsigned epoch)
+/* sol = base - D^-1 sum_s c_s ML_s X_s with X[s] the published trace w of
+ * slot s's neighbour; sol goes to fn and w = (1 - theta) fv + theta sol
+ * (fv: the lane's old value) to the hand-off words, each rounded once to
+ * R. S: the scratch of prep_compute, free again here. */
+template <typename R>
+__device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], R fv,
+				    double (&S)[3][36], const SweepTables &T,
+				    R *__restrict__ fn, u64 *ll, int ne, unsigned epoch)
 {
 	const int t = threadIdx.x;
 	const int iw = t / 10, i = t % 10;
+	double(*Y)[10] = reinterpret_cast<double(*)[10]>(&S[0][0]);
 	if (t < 60) {
-		const long bq = (long)(iw * nv + B.v) * neq;
-		double s = B.base[iw][i];
+		double y = 0.0;
+		for (int s = 0; s < B.nslot; s++) {
+			const int f = B.fsl[s];
+			double a = 0.0;
 #pragma unroll
-		for (int sl = 0; sl < 3; sl++) {
-			if (sl >= B.nslot)
-				break;
-			if (PERSIST) {
-#pragma unroll
-				for (int b = 0; b < 6; b++)
-					s -= B.G[sl][i][b] * X[sl][iw][b];
-			} else {
-				const long g0 = bq + 10L * B.nb[sl];
-#pragma unroll
-				for (int b = 0; b < 6; b++)
-					s -= B.G[sl][i][b] *
-					     (double)fn[g0 + ((B.map[sl] >> (4 * b)) & 15)];
-			}
+			for (int b = 0; b < 6; b++)
+				a += T.ML[f][i][b] * X[s][6 * iw + b];
+			y += B.cs[s] * a;
 		}
-		const R sr = (R)s;
-		fn[bq + 10L * B.e + i] = sr;
-		if (PERSIST)
-			Handoff<R>::put(ll + Handoff<R>::words * (((long)B.v * ne + B.e) * 60 + t),
-					sr, epoch);
+		Y[iw][i] = y;
+	}
+	__syncthreads();
+	if (t < 60) {
+		double a = 0.0;
+#pragma unroll
+		for (int l = 0; l < 10; l++)
+			a += B.D[i][l] * Y[iw][l];
+		const double sol = B.base[iw][i] - a;
+		fn[((long)B.v * ne + B.e) * 60 + t] = (R)sol;
+		Handoff<R>::put(ll_item<R>(ll, B.v, B.e, ne), t,
+				(R)((1.0 - THETA) * (double)fv + THETA * sol), epoch);
 	}
 	__syncthreads();
 }
@@ -505,21 +561,47 @@ __device__ inline void load_tables(SweepTables &T)
 	__syncthreads();
 }
 
-/* One upwind level per launch, one item per 64-lane block. */
+/* One upwind level per launch, one item per 64-lane block; the inflow
+ * traces are the hand-off words of earlier levels (a word without the
+ * sweep's epoch is reported in *status). */
 template <typename R>
 __global__ __launch_bounds__(64) void k_sweep_level(
 	const ItemDesc *__restrict__ desc, int begin, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
-	int nv, int ne, int neq)
+	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
+	const unsigned *__restrict__ epoch_dev, int *status)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B;
-	__shared__ double Z[6][10];
+	__shared__ double S[3][36];
+	__shared__ double X[3][36];
 	load_tables(T);
+	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
 	const int it = begin + blockIdx.x;
 	const ItemDesc d = desc[it];
-	sweep_prepare(it, d, fo, xl, dinv, ne, neq, nv, T, B, Z);
-	sweep_finish<false, R>(B, nullptr, fn, nullptr, nv, ne, neq, 0);
+	PrepLoads<R> P;
+	prep_issue(it, d, fo, dinv, ne, neq, nv, P);
+	prep_compute(d, P, xl, ne, neq, nv, T, B, S);
+	const int t = threadIdx.x;
+	const unsigned pending =
+		ll_poll<R>(B, ll, ne, epoch, t < 36 ? (1u << B.nslot) - 1u : 0u, X);
+	if (__ballot(pending != 0) && t == 0)
+		__hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__syncthreads();
+	sweep_finish<R>(B, X, P.fv, S, T, fn, ll, ne, epoch);
+}
+
+/* an ItemDesc held one dword per lane (lane k: dword k), made uniform */
+__device__ inline ItemDesc desc_from_lanes(int x)
+{
+	static_assert(sizeof(ItemDesc) == 80, "ItemDesc is 20 dwords");
+	int w[20];
+#pragma unroll
+	for (int k = 0; k < 20; k++)
+		w[k] = __builtin_amdgcn_readlane(x, k);
+	ItemDesc d;
+	__builtin_memcpy(&d, w, sizeof d);
+	return d;
 }
 
 /*
@@ -528,14 +610,17 @@ __global__ __launch_bounds__(64) void k_sweep_level(
  * (always earlier items). All G blocks are co-resident, so the earliest
  * unfinished item always has its inputs ready and the sweep cannot
  * deadlock; a bounded spin reports a timeout in *status instead of hanging.
- * While an item's inputs are not ready, the block prepares its next item.
+ * The loads that prepare the next item are issued before the current item's
+ * poll (one round trip for both), and while the current item's inputs are
+ * not ready the block prepares the next item. Item descriptors are read one
+ * item further ahead.
  *
  * With disc != nullptr the launch only measures residency: every block
  * records a 100 us busy interval, and the largest number of overlapping
  * intervals is a set of blocks that were resident together.
  */
-template <typename R>
-__global__ __launch_bounds__(64) void k_sweep_persistent(
+template <typename R, bool FULL_POLL>
+__global__ __launch_bounds__(64, 6) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
 	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
@@ -543,8 +628,8 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
-	__shared__ double Z[6][10];
-	__shared__ double X[3][6][6];
+	__shared__ double S[3][36];
+	__shared__ double X[3][36];
 	const int t = threadIdx.x;
 	if (disc) {
 		if (t == 0) {
@@ -560,32 +645,47 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 	load_tables(T);
 	/* the sweep's epoch: an argument, or (captured graphs) device memory */
 	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
+	const int G = gridDim.x;
 	int it = blockIdx.x, k = 0;
-	if (it < nitems)
-		sweep_prepare(it, desc[it], fo, xl, dinv, ne, neq, nv, T, B[0], Z);
-	for (; it < nitems; it += gridDim.x, k ^= 1) {
-		const int next = it + gridDim.x;
-		ItemDesc dn;
-		if (next < nitems)
-			dn = desc[next];
-		/* one full pass; if something is missing, prepare the next item
-		 * and then watch one word per slot */
+	if (it >= nitems)
+		return;
+	R fv; /* the lane's old value of the current item */
+	{
+		const ItemDesc d = desc[it];
+		PrepLoads<R> P;
+		prep_issue(it, d, fo, dinv, ne, neq, nv, P);
+		prep_compute(d, P, xl, ne, neq, nv, T, B[0], S);
+		fv = P.fv;
+	}
+	ItemDesc dn; /* descriptor of item it + G */
+	if (it + G < nitems)
+		dn = desc[it + G];
+	for (; it < nitems; it += G, k ^= 1) {
+		const int next = it + G;
+		const bool has_next = next < nitems;
+		PrepLoads<R> P;
+		if (has_next)
+			prep_issue(next, dn, fo, dinv, ne, neq, nv, P);
+		/* descriptor of item next + G: one dword per lane (t < 20) */
+		int dnn = 0;
+		if (next + G < nitems && t < (int)(sizeof(ItemDesc) / 4))
+			dnn = reinterpret_cast<const int *>(desc + next + G)[t];
 		unsigned pending = t < 36 ? (1u << B[k].nslot) - 1u : 0u;
 		pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 		if (__ballot(pending != 0)) {
-			bool prepared = false;
-			if (next < nitems) {
-				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
-				prepared = true;
-			}
+			/* prepare the next item, then watch one word per slot */
+			if (has_next)
+				prep_compute(dn, P, xl, ne, neq, nv, T, B[k ^ 1], S);
 			int spins = 0;
-			bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
-			while (!ready) {
-				__builtin_amdgcn_s_sleep(2);
-				ready = ll_sentinels<R>(B[k], ll, ne, epoch) ||
-					spin_expired(++spins, status);
+			if (!FULL_POLL) {
+				bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
+				while (!ready) {
+					__builtin_amdgcn_s_sleep(2);
+					ready = ll_sentinels<R>(B[k], ll, ne, epoch) ||
+						spin_expired(++spins, status);
+				}
+				pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 			}
-			pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 			while (__ballot(pending != 0)) {
 				__builtin_amdgcn_s_sleep(1);
 				pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
@@ -593,15 +693,17 @@ __global__ __launch_bounds__(64) void k_sweep_persistent(
 					break;
 			}
 			__syncthreads(); /* X complete */
-			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
-			if (next < nitems && !prepared)
-				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+			sweep_finish<R>(B[k], X, fv, S, T, fn, ll, ne, epoch);
 		} else {
 			__syncthreads(); /* X complete */
-			sweep_finish<true, R>(B[k], X, fn, ll, nv, ne, neq, epoch);
-			if (next < nitems)
-				sweep_prepare(next, dn, fo, xl, dinv, ne, neq, nv, T, B[k ^ 1], Z);
+			sweep_finish<R>(B[k], X, fv, S, T, fn, ll, ne, epoch);
+			if (has_next)
+				prep_compute(dn, P, xl, ne, neq, nv, T, B[k ^ 1], S);
 		}
+		if (has_next)
+			fv = P.fv;
+		if (next + G < nitems)
+			dn = desc_from_lanes(dnn);
 	}
 }
 
@@ -616,11 +718,11 @@ template <typename R> static int sweep_residency(gdh_ctx *c)
 	int dev = 0, ncu = 0, per_cu = 0;
 	HIPCHECK(hipGetDevice(&dev));
 	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<R>,
+	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<R, false>,
 							      64, 0));
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
-	k_sweep_persistent<R><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
+	k_sweep_persistent<R, false><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
 							nullptr, nullptr, 0, 0, 0, nullptr, 0,
 							nullptr, nullptr, disc);
 	HIPCHECK(hipGetLastError());
@@ -665,6 +767,10 @@ void sweep_setup(gdh_ctx *c)
 	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 	const char *mode = getenv("GDN_SWEEP");
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
+	/* waiting items poll one word per inflow face, then the traces
+	 * (default), or the whole traces on every pass (GDN_SWEEP_POLL=full) */
+	const char *poll = getenv("GDN_SWEEP_POLL");
+	c->sweep_poll = (poll && !strcmp(poll, "full")) ? 1 : 0;
 }
 
 __global__ void k_epoch_bump(unsigned *e)
@@ -675,51 +781,52 @@ __global__ void k_epoch_bump(unsigned *e)
 template <typename R>
 static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 {
-	if (c->sweep_mode == 0) {
-		for (int L = 0; L < c->nlevels; L++) {
-			const int b = c->level_ptr[L], n = c->level_ptr[L + 1] - b;
-			if (!n)
-				continue;
-			c->prof_launches++;
-			k_sweep_level<R><<<n, 64, 0, c->stream>>>(c->desc, b, fo, fn, xl,
-								  c->dinv, c->kp.nv, c->ne,
-								  c->neq);
-		}
-		HIPCHECK(hipGetLastError());
-		return;
-	}
-	c->prof_launches++;
-	const ItemDesc *desc = c->desc;
-	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
-	const double *dinv = c->dinv;
-	u64 *ll = c->ll, *disc = nullptr;
-	int *status = c->status;
-	const R *fo_c = fo;
-	const int grid = std::min(c->sweep_grid, nitems);
+	/* the sweep's epoch: on the host, or (batches, captured in graphs) in
+	 * device memory advanced by a one-thread node */
 	if (c->epoch_dev) {
-		/* batches (graph capture): a one-thread node advances the epoch in
-		 * device memory; an ordinary launch with the measured resident
-		 * grid, since a captured stream takes no cooperative launch */
 		k_epoch_bump<<<1, 1, 0, c->stream>>>(c->epoch_dev);
-		k_sweep_persistent<R><<<grid, 64, 0, c->stream>>>(
-			desc, nitems, fo_c, fn, xl, dinv, nv, ne, neq, ll, 0u, c->epoch_dev,
-			status, disc);
-		HIPCHECK(hipGetLastError());
-		return;
-	}
-	if (++c->epoch == 0) {
+	} else if (++c->epoch == 0) {
 		/* 2^32 sweeps: restart the epochs from a zeroed buffer */
 		HIPCHECK(hipMemsetAsync(c->ll, 0,
 					sizeof(u64) * Handoff<R>::words * 60 * (size_t)c->nitems,
 					c->stream));
 		c->epoch = 1;
 	}
+	const ItemDesc *desc = c->desc;
+	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
+	const double *dinv = c->dinv;
+	u64 *ll = c->ll, *disc = nullptr;
+	int *status = c->status;
 	unsigned epoch = c->epoch;
-	const unsigned *edev = nullptr;
+	const unsigned *edev = c->epoch_dev;
+	if (c->sweep_mode == 0) {
+		for (int L = 0; L < c->nlevels; L++) {
+			const int b = c->level_ptr[L], n = c->level_ptr[L + 1] - b;
+			if (!n)
+				continue;
+			c->prof_launches++;
+			k_sweep_level<R><<<n, 64, 0, c->stream>>>(desc, b, fo, fn, xl, dinv, nv, ne,
+								  neq, ll, epoch, edev, status);
+		}
+		HIPCHECK(hipGetLastError());
+		return;
+	}
+	c->prof_launches++;
+	const R *fo_c = fo;
+	const int grid = std::min(c->sweep_grid, nitems);
+	auto kern = c->sweep_poll ? k_sweep_persistent<R, true> : k_sweep_persistent<R, false>;
+	if (edev) {
+		/* an ordinary launch with the measured resident grid, since a
+		 * captured stream takes no cooperative launch */
+		kern<<<grid, 64, 0, c->stream>>>(desc, nitems, fo_c, fn, xl, dinv, nv, ne, neq, ll,
+						 0u, edev, status, disc);
+		HIPCHECK(hipGetLastError());
+		return;
+	}
 	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
 			 &epoch, &edev, &status, &disc };
-	HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_persistent<R>, dim3(grid),
-					    dim3(64), args, 0, c->stream));
+	HIPCHECK(hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(64), args, 0,
+					    c->stream));
 }
 
 void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl)
@@ -730,14 +837,14 @@ void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl)
 		sweep_step_t<double>(c, (double *)fo, (double *)fn, (const double *)xl);
 }
 
-/* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [0, n)
- * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54); NV = kp.nv,
- * fixed at compile time so that the node's 6 NV values stay in registers. */
-template <int NV, typename R, int BS>
-__global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp,
-					      long n, int neq)
+/* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [ipg0, n)
+ * (src/dg/dg.c:109-132, src/simulation/simulation.c:42-54), one DoF per
+ * thread (the ghost DoFs); NV = kp.nv. */
+template <int NV, typename R>
+__global__ __launch_bounds__(256) void k_relax_dof(R *__restrict__ f, KinPar kp,
+						   long ipg0, long n, long ne, long neq)
 {
-	const long ipg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long ipg = ipg0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (ipg >= n)
 		return;
 	double fo[6][NV];
@@ -747,7 +854,7 @@ __global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp,
 		double s = 0.0;
 #pragma unroll
 		for (int v = 0; v < NV; v++) {
-			const double x = (double)f[(long)(iw * NV + v) * neq + ipg];
+			const double x = (double)f[kin_at(ne, neq, NV, ipg, iw, v)];
 			fo[iw][v] = x;
 			s += x;
 		}
@@ -760,17 +867,99 @@ __global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp,
 	for (int iw = 0; iw < 6; iw++)
 #pragma unroll
 		for (int v = 0; v < NV; v++)
-			f[(long)(iw * NV + v) * neq + ipg] =
+			f[kin_at(ne, neq, NV, ipg, iw, v)] =
 				(R)(om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v]);
+}
+
+/* row iw of maxwell_qM(w), by selects (iw differs between lanes) */
+__device__ inline void maxwell_q_row(const double w[6], int iw, double r[3])
+{
+	double q[6][3];
+	maxwell_qM(w, q);
+	r[0] = r[1] = r[2] = 0.0;
+#pragma unroll
+	for (int k = 0; k < 6; k++)
+		if (iw == k) {
+			r[0] = q[k][0];
+			r[1] = q[k][1];
+			r[2] = q[k][2];
+		}
+}
+
+/*
+ * The same relaxation over the element DoFs, one 64-lane wave per element:
+ * lane t = 10 iw + i (t < 60) holds value (iw, i) of every velocity, so each
+ * velocity's 60 values are one contiguous load of the item-major state.
+ * The node's six w components come from the lanes i, 10 + i, ... by
+ * shuffles. BS / 64 elements per block.
+ */
+template <int NV, typename R, int BS>
+__global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp, long ne)
+{
+	const long e = (long)blockIdx.x * (BS / 64) + threadIdx.x / 64;
+	if (e >= ne)
+		return; /* whole waves */
+	const int t = threadIdx.x % 64;
+	const int tt = t < 60 ? t : 59, iw = tt / 10, i = tt % 10;
+	R *p = f + e * 60 + tt;
+	const long vs = 60 * ne;
+	double fo[NV];
+	double s = 0.0;
+#pragma unroll
+	for (int v = 0; v < NV; v++) {
+		fo[v] = (double)p[v * vs];
+		s += fo[v];
+	}
+	double w[6];
+#pragma unroll
+	for (int q = 0; q < 6; q++)
+		w[q] = __shfl(s, 10 * q + i, 64);
+	double r[3];
+	maxwell_q_row(w, iw, r);
+	const double om = kp.omega;
+	double out[NV];
+	if (kp.kind == GDH_KIN_D3Q4) {
+		/* feq_comp, D3Q4 branch */
+		const double l2 = kp.lambda * kp.lambda;
+#pragma unroll
+		for (int v = 0; v < NV; v++) {
+			const double *vv = kp.vi + 3 * v;
+			const double qv = r[0] * vv[0] + r[1] * vv[1] + r[2] * vv[2];
+			out[v] = om * ((s + qv / l2) / kp.nv) + (1. - om) * fo[v];
+		}
+	} else {
+		/* feq_comp, D3Q6 branch: v = +-lambda e_k, index k or 3 + k */
+#pragma unroll
+		for (int v = 0; v < NV; v++) {
+			const double a = s / 6., b = r[v % 3] / (2. * kp.lambda);
+			out[v] = om * (v < 3 ? a + b : a - b) + (1. - om) * fo[v];
+		}
+	}
+	if (t < 60)
+#pragma unroll
+		for (int v = 0; v < NV; v++)
+			p[v * vs] = (R)out[v];
+}
+
+template <int NV, typename R, int BS> static void relax_launch_nv(gdh_ctx *c, R *f, long n)
+{
+	k_relax<NV, R, BS><<<nblocks(c->ne, BS / 64), BS, 0, c->stream>>>(f, c->kp, c->ne);
+	const long g0 = 10L * c->ne;
+	if (n > g0)
+		k_relax_dof<NV, R><<<nblocks(n - g0, 256), 256, 0, c->stream>>>(f, c->kp, g0, n,
+										  c->ne, c->neq);
 }
 
 template <typename R, int BS> static void relax_launch_bs(gdh_ctx *c, R *f, long n)
 {
-	const int nb = nblocks(n, BS);
+	if (n < 10L * c->ne) {
+		fprintf(stderr, "gdon3d-hip: relaxation of part of the elements\n");
+		abort();
+	}
 	if (c->kp.nv == 4)
-		k_relax<4, R, BS><<<nb, BS, 0, c->stream>>>(f, c->kp, n, c->neq);
+		relax_launch_nv<4, R, BS>(c, f, n);
 	else if (c->kp.nv == 6)
-		k_relax<6, R, BS><<<nb, BS, 0, c->stream>>>(f, c->kp, n, c->neq);
+		relax_launch_nv<6, R, BS>(c, f, n);
 	else {
 		fprintf(stderr, "gdon3d-hip: no relaxation kernel for %d velocities\n",
 			c->kp.nv);
@@ -809,14 +998,14 @@ void relax_launch(gdh_ctx *c, void *f, long n)
  */
 template <typename R>
 __global__ void k_cap_gather(const R *__restrict__ x, const int *__restrict__ ipg,
-			     const int *__restrict__ vel, int q, int nv, int neq,
+			     const int *__restrict__ vel, int q, int nv, long ne, long neq,
 			     double *__restrict__ t)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= 6 * q)
 		return;
 	const int iw = k / q, r = k % q;
-	t[k] = (double)x[(long)(iw * nv + vel[r]) * neq + ipg[r]];
+	t[k] = (double)x[kin_at(ne, neq, nv, ipg[r], iw, vel[r])];
 }
 
 /* z = (I - T S)_v^-1 t for one (v, iw) per block: LU column-major with row
@@ -826,7 +1015,7 @@ __global__ __launch_bounds__(64) void k_cap_solve(
 	const double *__restrict__ lu, const long *__restrict__ luoff,
 	const int *__restrict__ piv, const int *__restrict__ off,
 	const int *__restrict__ ipg, const double *__restrict__ t, int qtot, int nv,
-	int neq, R *__restrict__ wz)
+	long ne, long neq, R *__restrict__ wz)
 {
 	extern __shared__ double y[];
 	const int v = blockIdx.x / 6, iw = blockIdx.x % 6;
@@ -853,21 +1042,21 @@ __global__ __launch_bounds__(64) void k_cap_solve(
 		__syncthreads();
 	}
 	for (int i = threadIdx.x; i < q; i += 64)
-		wz[(long)(iw * nv + v) * neq + ipg[o + i]] = (R)y[i];
+		wz[kin_at(ne, neq, nv, ipg[o + i], iw, v)] = (R)y[i];
 }
 
 /* unit lagged inputs of set-up batch c: field slot iw carries column
  * 6 c + iw of every velocity */
 template <typename R>
 __global__ void k_cap_unit(R *__restrict__ wz, const int *__restrict__ ipg,
-			   const int *__restrict__ off, int batch, int nv, int neq)
+			   const int *__restrict__ off, int batch, int nv, long ne, long neq)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= 6 * nv)
 		return;
 	const int v = k / 6, iw = k % 6, r = 6 * batch + iw;
 	if (r < off[v + 1] - off[v])
-		wz[(long)(iw * nv + v) * neq + ipg[off[v] + r]] = (R)1;
+		wz[kin_at(ne, neq, nv, ipg[off[v] + r], iw, v)] = (R)1;
 }
 
 static void cap_gather(gdh_ctx *c, const void *x)
@@ -875,10 +1064,10 @@ static void cap_gather(gdh_ctx *c, const void *x)
 	const int q = c->cap_qtot;
 	if (c->fp32)
 		k_cap_gather<float><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
-			(const float *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->neq, c->cap_t);
+			(const float *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->ne, c->neq, c->cap_t);
 	else
 		k_cap_gather<double><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
-			(const double *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->neq, c->cap_t);
+			(const double *)x, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->ne, c->neq, c->cap_t);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -976,10 +1165,10 @@ static void cap_setup(gdh_ctx *c)
 		HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
 		if (c->fp32)
 			k_cap_unit<float><<<nblocks(6 * nv, 64), 64, 0, c->stream>>>(
-				(float *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->neq);
+				(float *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->ne, c->neq);
 		else
 			k_cap_unit<double><<<nblocks(6 * nv, 64), 64, 0, c->stream>>>(
-				(double *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->neq);
+				(double *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->ne, c->neq);
 		HIPCHECK(hipGetLastError());
 		sweep_step(c, zero, out, c->wz);
 		cap_gather(c, out);
@@ -1027,11 +1216,11 @@ static void cap_solve(gdh_ctx *c)
 	if (c->fp32)
 		k_cap_solve<float><<<6 * nv, 64, lds, c->stream>>>(
 			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
-			c->cap_qtot, nv, c->neq, (float *)c->wz);
+			c->cap_qtot, nv, c->ne, c->neq, (float *)c->wz);
 	else
 		k_cap_solve<double><<<6 * nv, 64, lds, c->stream>>>(
 			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
-			c->cap_qtot, nv, c->neq, (double *)c->wz);
+			c->cap_qtot, nv, c->ne, c->neq, (double *)c->wz);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -1068,6 +1257,11 @@ extern "C" void gdh_apply_relaxation(gdh_ctx *c)
 	relax_launch(c, c->w[c->cur], n);
 }
 
+
+extern "C" int gdh_sweep_grid(gdh_ctx *c)
+{
+	return c->sweep_grid;
+}
 
 extern "C" void gdh_transport_stats(gdh_ctx *c, int out[3])
 {

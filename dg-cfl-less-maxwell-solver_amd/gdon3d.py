@@ -173,6 +173,7 @@ def load_library(path=LIB_PATH):
                                        C.c_int, C.c_float, C.c_uint64, c_double_p]),
         "gdh_mlp_predict": (None, [C.c_void_p, c_float_p, C.c_int, c_float_p]),
         "gdh_learned_data": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_float_p, c_float_p]),
+        "gdh_sweep_grid": (C.c_int, [C.c_void_p]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
@@ -462,6 +463,12 @@ class BlockNet:
         if self.h:
             self.lib.gdh_mlp_destroy(self.h)
             self.h = None
+
+
+def sweep_grid(stepper):
+    """Workgroups of the one-launch transport sweep (measured residency)."""
+    lib = load_library()
+    return lib.gdh_sweep_grid(lib.dg_stepper_device_context(stepper.h))
 
 
 def learned_data(stepper, idx):

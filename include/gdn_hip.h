@@ -168,6 +168,10 @@ double gdh_timer_stop(gdh_ctx *ctx);
  * lagged inputs (built and LU-factored once at set-up): two sweeps per step. */
 void gdh_transport_stats(gdh_ctx *ctx, int out[3]);
 
+/* Workgroups of the one-launch transport sweep: the number measured to be
+ * resident together at set-up (no counterpart in the reference). */
+int gdh_sweep_grid(gdh_ctx *ctx);
+
 /* ---- batches (no counterpart in the reference: one drop-in simulation
  * per call there). The context's mesh is the disjoint union of ninst
  * instance meshes: instance i owns elements [elem_first[i], elem_first[i+1])
