@@ -85,9 +85,11 @@ def test_fp32_relaxation_cavity_d3q6():
 
 
 @pytest.mark.gpu
-def test_fp32_persistent_sweep_equals_level_sweep(monkeypatch):
+@pytest.mark.parametrize("poll", ["sentinel", "full"])
+def test_fp32_persistent_sweep_equals_level_sweep(poll, monkeypatch):
     path = kuhn_msh(6, 0.2, 31)
     G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    monkeypatch.setenv("GDN_SWEEP_POLL", poll)
     out = {}
     for mode in ("level", "persistent"):
         monkeypatch.setenv("GDN_SWEEP", mode)

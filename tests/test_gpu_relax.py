@@ -63,14 +63,18 @@ def test_relax_d3q6():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("poll", ["sentinel", "full"])
 @pytest.mark.parametrize("n,jitter,tmax", [(4, 0.0, 0.6), (6, 0.2, 0.6), (16, 0.1, 0.1)])
-def test_persistent_sweep_equals_level_sweep(n, jitter, tmax, monkeypatch):
+def test_persistent_sweep_equals_level_sweep(n, jitter, tmax, poll, monkeypatch):
     """The one-launch dataflow sweep (epoch-tagged hand-off between
     workgroups) computes bit for bit what the level-by-level launches
-    compute. At 16^3 (98,304 items) every resident block processes about 20
-    items, so the in-order item pipeline of each block is exercised."""
+    compute, with either way of waiting (one word per inflow face, then the
+    traces; or the whole traces on every pass). At 16^3 (98,304 items)
+    every resident block processes about 20 items, so the in-order item
+    pipeline of each block is exercised."""
     path = kuhn_msh(n, jitter, 31)
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    monkeypatch.setenv("GDN_SWEEP_POLL", poll)
     out = {}
     for mode in ("level", "persistent"):
         monkeypatch.setenv("GDN_SWEEP", mode)
