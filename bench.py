@@ -40,7 +40,7 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # bytes, WRITE_SIZE all of them); per element of the workloads measured there
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
-SWEEP_TRAFFIC_BYTES_PER_ELEM = 6.245e9 / 196608
+SWEEP_TRAFFIC_BYTES_PER_ELEM = (1417840.5 / 0.5 + 1108374.3) * 1024 / 196608
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (2 * 2634964.1 + 1474560.0) * 1024 / 1572864
 
 
