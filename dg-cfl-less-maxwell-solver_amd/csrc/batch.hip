@@ -115,28 +115,32 @@ template <typename F> static float time_launches(gdh_ctx *c, int reps, F f)
 }
 
 /*
- * Tile sizes by measurement on the batch: the RK3 stage with 64, 128 or 256
- * elements per block (first-stage launches from the current state into the
- * other buffer, before the run starts), the relaxation with 128, 256 or 512
- * nodes per block (on the other buffer). out = { rk3_bs, relax_bs }.
+ * Tile sizes by measurement on the batch: the RK3 stage with 64 elements
+ * per block staged in LDS, or 128 or 256 elements per block over the
+ * global state (first-stage launches from the current state into the other
+ * buffer, before the run starts), the relaxation with 128, 256 or 512
+ * nodes per block (on the other buffer). out = { rk3 elements per block,
+ * relax nodes per block }.
  */
 extern "C" void gdh_batch_autotune(gdh_ctx *c, int out[2])
 {
 	const int prof = c->prof;
 	c->prof = 0;
 	if (!c->scheme) {
-		static const int cand[3] = { 64, 128, 256 };
+		static const int cand[3][2] = { { 64, 1 }, { 128, 0 }, { 256, 0 } };
 		float best = 1e30f;
-		int pick = 128;
-		for (int bs : cand) {
-			c->rk3_bs = bs;
+		int pick = 0;
+		for (int k = 0; k < 3; k++) {
+			c->rk3_bs = cand[k][0];
+			c->rk3_tile = cand[k][1];
 			const float ms = time_launches(c, 5, [&] { rk3_trial(c); });
 			if (ms < best) {
 				best = ms;
-				pick = bs;
+				pick = k;
 			}
 		}
-		c->rk3_bs = pick;
+		c->rk3_bs = cand[pick][0];
+		c->rk3_tile = cand[pick][1];
 	} else {
 		static const int cand[3] = { 128, 256, 512 };
 		float best = 1e30f;

@@ -224,6 +224,13 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	}
 	c->graph[0] = c->graph[1] = nullptr;
 	c->rk3_bs = 128;
+	{
+		/* RK3 stage kernel: the LDS-tiled one (default), or one element per
+		 * thread over the global state (GDN_RK3=soa; both compute the same
+		 * arithmetic in the same order) */
+		const char *k = getenv("GDN_RK3");
+		c->rk3_tile = !(k && !strcmp(k, "soa"));
+	}
 	c->relax_bs = 256;
 	c->cap_qtot = 0;
 	c->cap_ipg = c->cap_vel = c->cap_off = c->cap_piv = nullptr;

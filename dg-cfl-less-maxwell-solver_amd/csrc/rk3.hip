@@ -157,6 +157,63 @@ __device__ __forceinline__ void face_term(T (&acc)[6][10], const T *__restrict__
 	}
 }
 
+/* face_term with the tile's values in LDS (Ws, column t) and the neighbour
+ * trace from LDS when the neighbour column is in the tile [c0, c0 + 64) */
+template <int F, typename T>
+__device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
+					       const T *__restrict__ W,
+					       const T *__restrict__ GH, long ne, long ngh,
+					       int t, int c0, int nb, int map,
+					       const double (&ctd)[9], int bflux,
+					       const Rk3Tables<T> *__restrict__ Tb)
+{
+	double vnd[3];
+	face_normal(ctd, F, vnd);
+	const T overr = (T)(1.0 / (sqrt(vnd[0] * vnd[0] + vnd[1] * vnd[1] +
+					vnd[2] * vnd[2]) + 1e-16));
+	const T vn[3] = { (T)vnd[0], (T)vnd[1], (T)vnd[2] };
+	/* the whole neighbour trace first, one load path for every lane (a
+	 * neighbour inside the tile was just read by this block and comes from
+	 * the cache): one round trip per face */
+	(void)c0;
+	T wra[6][6];
+#pragma unroll
+	for (int b = 0; b < 6; b++) {
+		const int j = (map >> (4 * b)) & 15;
+		if (nb >= 0) {
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[b][q] = W[(q * 10 + j) * ne + nb];
+		} else {
+			const long k = 6L * (-1 - nb) + b;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[b][q] = GH[q * ngh + k];
+		}
+	}
+#pragma unroll
+	for (int b = 0; b < 6; b++) {
+		const int i = face_node(F, b);
+		T wl[6], fl[6];
+		T(&wr)[6] = wra[b];
+#pragma unroll
+		for (int q = 0; q < 6; q++)
+			wl[q] = Ws[(q * 10 + i) * 64 + t];
+		if (nb < 0 && bflux == GDH_FLUX_MAXWELL_METAL)
+			maxwell_metal(wl, vn, fl);
+		else
+			maxwell_upwind(wl, wr, vn, overr, fl);
+#pragma unroll
+		for (int m = 0; m < 10; m++) {
+			const T lift = Tb->minv_lift[F][m][b];
+#pragma unroll
+			for (int p = 0; p < 6; p++)
+				acc[p][m] += lift * fl[p];
+		}
+		pin(acc);
+	}
+}
+
 /* One stage over storage columns e; T = double or float for the state and
  * the flux/volume arithmetic, geometry G always FP64; BS elements per block.
  * Batches: the time step of the column's instance (dt_tab[inst[e]]) and,
@@ -227,6 +284,93 @@ __global__ __launch_bounds__(BS) void k_rk3_soa(
 		}
 }
 
+/*
+ * The same stage with the state of a tile of 64 consecutive storage columns
+ * (a compact Morton block) staged in LDS by one coalesced pass: the
+ * element's own values are read from LDS in every phase, and so is the
+ * trace of a face neighbour inside the tile (about 70 % of the interior
+ * faces at 64 elements per tile); neighbours outside the tile and ghosts
+ * come from global memory. One 64-lane block per tile, one element per
+ * lane; the arithmetic is that of k_rk3_soa.
+ */
+template <typename T>
+__global__ __launch_bounds__(64) void k_rk3_tile(
+	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
+	const T *__restrict__ GH, const double *__restrict__ G,
+	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
+	const Rk3Tables<T> *__restrict__ Tb, int ne_, int ngh_, double Ac,
+	double Bc, double dt, int first, int bflux, const int *__restrict__ inst,
+	const double *__restrict__ dt_tab, const double *__restrict__ ieps)
+{
+	__shared__ T Ws[60 * 64]; /* value (q, i) of tile column t at (10 q + i) 64 + t */
+	const long ne = ne_, ngh = ngh_;
+	const int tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int t = threadIdx.x, c0 = tile * 64, e = c0 + t;
+	const bool live = e < ne_;
+	{
+		/* all 60 loads in flight, then the LDS stores */
+		T v[60];
+#pragma unroll
+		for (int r = 0; r < 60; r++)
+			v[r] = live ? W[r * ne + e] : T(0);
+#pragma unroll
+		for (int r = 0; r < 60; r++)
+			Ws[r * 64 + t] = v[r];
+	}
+	__syncthreads();
+	if (!live)
+		return;
+	double ctd[9];
+#pragma unroll
+	for (int k = 0; k < 9; k++)
+		ctd[k] = G[k * ne + e];
+	const double det = G[9 * ne + e];
+
+	T acc[6][10];
+#pragma unroll
+	for (int p = 0; p < 6; p++)
+#pragma unroll
+		for (int m = 0; m < 10; m++)
+			acc[p][m] = T(0);
+
+	volume_field<0>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+	volume_field<1>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+	volume_field<2>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+	volume_field<3>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+	volume_field<4>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+	volume_field<5>(acc, Ws, 64, t, ctd, Tb);
+	pin(acc);
+
+	const int4 n4 = nbr[e], m4 = nbmap[e];
+	face_term_tile<0>(acc, Ws, W, GH, ne, ngh, t, c0, n4.x, m4.x, ctd, bflux, Tb);
+	pin(acc);
+	face_term_tile<1>(acc, Ws, W, GH, ne, ngh, t, c0, n4.y, m4.y, ctd, bflux, Tb);
+	pin(acc);
+	face_term_tile<2>(acc, Ws, W, GH, ne, ngh, t, c0, n4.z, m4.z, ctd, bflux, Tb);
+	pin(acc);
+	face_term_tile<3>(acc, Ws, W, GH, ne, ngh, t, c0, n4.w, m4.w, ctd, bflux, Tb);
+	pin(acc);
+
+	const T invdet = (T)(1.0 / det);
+	const T idE = ieps ? (T)(ieps[e] / det) : invdet;
+	const T dtT = (T)(inst ? dt_tab[inst[e]] : dt), AcT = (T)Ac, BcT = (T)Bc;
+#pragma unroll
+	for (int q = 0; q < 6; q++)
+#pragma unroll
+		for (int i = 0; i < 10; i++) {
+			const long g = (q * 10 + i) * ne + e;
+			const T d = acc[q][i] * (q < 3 ? idE : invdet);
+			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
+			K2[g] = k;
+			Wo[g] = Ws[(q * 10 + i) * 64 + t] + BcT * k;
+		}
+}
+
 template <typename T, int BS>
 static void rk3_launch_bs(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 			  double dt, int first, const void *W, void *Wo)
@@ -242,6 +386,14 @@ template <typename T>
 static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 		       double dt, int first, const void *W, void *Wo)
 {
+	if (c->rk3_tile) {
+		k_rk3_tile<T><<<nblocks(c->ne, 64), 64, 0, c->stream>>>(
+			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
+			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
+			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
+		HIPCHECK(hipGetLastError());
+		return;
+	}
 	switch (c->rk3_bs) {
 	case 64: rk3_launch_bs<T, 64>(c, tab, Ac, Bc, dt, first, W, Wo); break;
 	case 256: rk3_launch_bs<T, 256>(c, tab, Ac, Bc, dt, first, W, Wo); break;
