@@ -41,7 +41,7 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
 SWEEP_TRAFFIC_BYTES_PER_ELEM = (1417840.5 / 0.5 + 1108374.3) * 1024 / 196608
-RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (2 * 2634964.1 + 1474560.0) * 1024 / 1572864
+RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (1493823.2 / 0.5 + 1474560.0) * 1024 / 1572864
 
 
 def cpu_baseline(n, steps):
@@ -123,8 +123,11 @@ def bench_rk3(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem
-                                 if args.precision == "fp64" and n == 64 else None),
-                     "kernel": "k_rk3_soa", "bytes_per_element_stage": bytes_elem,
+                                 if args.precision == "fp64" and n == 64
+                                 and os.environ.get("GDN_RK3") != "soa" else None),
+                     "kernel": ("k_rk3_soa" if os.environ.get("GDN_RK3") == "soa"
+                                else "k_rk3_tile"),
+                     "bytes_per_element_stage": bytes_elem,
                      "avg_launch_us": stage_s * 1e6},
     }
     print(json.dumps(line), flush=True)
