@@ -239,7 +239,8 @@ struct SweepTables {
  * inflow face: its face index, coefficient c_f, neighbour and node map.
  */
 struct ItemBuf {
-	double G[3][10][6]; /* G_s = c_s D^-1 ML_s of slot s */
+	double G[3][10][7]; /* G_s = c_s D^-1 ML_s of slot s (rows padded: the
+			     * ten rows read at once fall in distinct banks) */
 	int v, e, nslot, pad;
 	int nb[3], map[3];
 };
@@ -247,7 +248,7 @@ struct ItemBuf {
 /* Per-block scratch of one item's preparation: D^-1, and 60 values (one
  * given trace at a time, then Z); the finish uses none of it. */
 struct PrepScratch {
-	double D[10][10];
+	double D[10][11]; /* D^-1, rows padded like G */
 	double S[60];
 };
 
@@ -351,9 +352,9 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
 	const int t = threadIdx.x;
 	const int gf = given_faces(d), sf = slot_faces(d);
 	const int iw = t / 10, i = t % 10;
-	(&W.D[0][0])[t] = P.d0;
+	W.D[t / 10][t % 10] = P.d0;
 	if (t < 36)
-		(&W.D[0][0])[64 + t] = P.d1;
+		W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
 	if (t == 0) {
 		int s = 0;
 		B.v = d.v;
