@@ -398,11 +398,13 @@ def main():
     st = G.Stepper(sim, precision=args.precision)
     st.sync()
     t_setup = time.perf_counter() - t0
+    relax_fused = G.relax_fused(st)
     sim_itermax = sim.s.itermax
     default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
                         and args.precision == "fp64")
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
-          f"levels={st.levels} sweep_grid={G.sweep_grid(st)} setup={t_setup:.2f}s",
+          f"levels={st.levels} sweep_grid={G.sweep_grid(st)} "
+          f"relax_fused={relax_fused} setup={t_setup:.2f}s",
           file=sys.stderr, flush=True)
 
     st.advance(args.warmup)
@@ -457,6 +459,7 @@ def main():
             "t_final": t_final,
             "l2_error": err,
             "hbm_gb_s_sweep": achieved,
+            "relaxation_fused_into_sweep": bool(relax_fused),
         },
         "roofline": {
             "bound": "hbm",

@@ -196,6 +196,13 @@ extern "C" void gdh_batch_advance(gdh_ctx *c, int nsteps)
 	}
 	const int prof = c->prof;
 	c->prof = 0; /* no event records inside captures */
+	/* kinetic: a captured step relaxes its input (the state awaits the
+	 * relaxation of the step before); a state that does not (fresh, or
+	 * materialized by a download) takes one direct step first */
+	if (c->scheme && c->relax_fused && !c->relax_pending && nsteps > 0) {
+		batch_step(c);
+		nsteps--;
+	}
 	const int pairs = nsteps / 2;
 	if (pairs > 0 && !c->graph[c->cur]) {
 		const int cur0 = c->cur;

@@ -119,6 +119,8 @@ struct gdh_ctx {
 	}
 	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
 	int sweep_poll; /* 1: waiting items poll whole traces */
+	int relax_pending; /* the state awaits the relaxation of its last step */
+	int relax_fused; /* the sweep applies the relaxation as it reads its input */
 
 	/* optional event timing of the main kernels: pairs of events recorded
 	 * on the stream around each kernel group, resolved when read */
@@ -228,7 +230,8 @@ __host__ __device__ inline long kin_at(long ne, long neq, int nv, long ipg, int 
 
 /* launchers implemented in rk3.hip and sweep.hip */
 void sweep_setup(gdh_ctx *c);
-void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl);
+void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl, int relax_in);
+void relax_materialize(gdh_ctx *c);
 void relax_launch(gdh_ctx *c, void *f, long n);
 void block_inverse_launch(gdh_ctx *c);
 

@@ -241,6 +241,8 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->status = nullptr;
 	c->nlevels = 0;
 	c->nitems = 0;
+	c->relax_pending = 0;
+	c->relax_fused = 0;
 	return c;
 }
 
@@ -331,6 +333,7 @@ template <typename T> static StateRef<T> state_ref(gdh_ctx *c)
 extern "C" void gdh_upload_state(gdh_ctx *c, const double *wn)
 {
 	const size_t len = (size_t)c->m * c->neq;
+	c->relax_pending = 0;
 	HIPCHECK(hipMemcpyAsync(c->stage, wn, sizeof(double) * len, hipMemcpyHostToDevice,
 				c->stream));
 	const int nb = nblocks(len, 256);
@@ -352,6 +355,7 @@ extern "C" void gdh_upload_state(gdh_ctx *c, const double *wn)
 
 extern "C" void gdh_download_state(gdh_ctx *c, double *wn)
 {
+	relax_materialize(c);
 	const size_t len = (size_t)c->m * c->neq;
 	const int nb = nblocks(len, 256);
 	if (c->scheme && c->fp32)
@@ -520,6 +524,7 @@ extern "C" void gdh_init_solution(gdh_ctx *c, const double *wexact)
 		abort();
 	}
 	const double *hv = stage_hostvals(c, wexact, 6 * n);
+	c->relax_pending = 0;
 	fill_launch(c, 0, n, c->xnode, hv, 0.0, 0.0);
 }
 
@@ -641,6 +646,7 @@ __global__ __launch_bounds__(64) void k_err_final(const double *__restrict__ par
 double error_range(gdh_ctx *c, int e0, int e1, double t, const ImposedPar &ip,
 		   const double *hv)
 {
+	relax_materialize(c);
 	const int np = std::min(c->npartials, std::max(1, nblocks(e1 - e0, ERR_BS)));
 	if (c->fp32)
 		k_err_partial<float><<<np, ERR_BS, 0, c->stream>>>(

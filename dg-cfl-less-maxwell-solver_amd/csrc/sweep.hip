@@ -15,6 +15,7 @@ using namespace gdh;
 #define CAP_MAX_NODES 4096 /* lagged nodes of one velocity (dense LU) */
 
 static void cap_setup(gdh_ctx *c);
+static void relax_choose(gdh_ctx *c);
 
 /* own outward flux coefficient s_f = v . vn(e, f) */
 __device__ inline double face_coef(const double ct[9], int f, const double *v)
@@ -156,6 +157,7 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 		c->wx = dmalloc(c->rsz * blen);
 		cap_setup(c);
 	}
+	relax_choose(c);
 }
 
 /* ------------------------------------------------------------------ */
@@ -228,6 +230,62 @@ __global__ void k_item_desc(const int2 *__restrict__ items, int nitems,
  * reference rounds it) */
 __constant__ static const double c_minv_lift[4][10][6] = T10_MINV_LIFT_INIT;
 
+/* row iw of maxwell_qM(w), by selects (iw differs between lanes) */
+__device__ inline void maxwell_q_row(const double w[6], int iw, double r[3])
+{
+	double q[6][3];
+	maxwell_qM(w, q);
+	r[0] = r[1] = r[2] = 0.0;
+#pragma unroll
+	for (int k = 0; k < 6; k++)
+		if (iw == k) {
+			r[0] = q[k][0];
+			r[1] = q[k][1];
+			r[2] = q[k][2];
+		}
+}
+
+/*
+ * The node-local relaxation f <- omega f_eq(w(f)) + (1 - omega) f of value
+ * (iw, i) = (t / 10, t % 10) of velocity v, from that value of every
+ * velocity (fr[0..NV-1]); the node's six w components come from lanes i,
+ * 10 + i, ... by shuffles, so every lane of the wave takes part. The same
+ * arithmetic as k_relax (src/dg/dg.c:109-132, src/simulation/simulation.c:
+ * 42-54).
+ */
+template <int NV, typename R>
+__device__ inline double relax_value(const KinPar &kp, const R (&fr)[NV], int v)
+{
+	const int t = threadIdx.x % 64;
+	const int tt = t < 60 ? t : 59, iw = tt / 10, i = tt % 10;
+	double s = 0.0, fo = 0.0;
+#pragma unroll
+	for (int u = 0; u < NV; u++) {
+		s += (double)fr[u];
+		fo = u == v ? (double)fr[u] : fo;
+	}
+	double w[6];
+#pragma unroll
+	for (int q = 0; q < 6; q++)
+		w[q] = __shfl(s, 10 * q + i, 64);
+	double r[3];
+	maxwell_q_row(w, iw, r);
+	/* the model's scalars taken where they are used (not values derived
+	 * once and held in vector registers across the item loop) */
+	double om = kp.omega, lam = kp.lambda;
+	int nvk = kp.nv;
+	asm volatile("" : "+s"(om), "+s"(lam), "+s"(nvk));
+	if (kp.kind == GDH_KIN_D3Q4) {
+		const double l2 = lam * lam;
+		const double *vv = kp.vi + 3 * v;
+		const double qv = r[0] * vv[0] + r[1] * vv[1] + r[2] * vv[2];
+		return om * ((s + qv / l2) / nvk) + (1. - om) * fo;
+	}
+	const int k = v % 3;
+	const double a = s / 6., b = r[k] / (2. * lam);
+	return om * (v < 3 ? a + b : a - b) + (1. - om) * fo;
+}
+
 /* Per-block operator table in LDS */
 struct SweepTables {
 	double ML[4][10][6];
@@ -254,12 +312,13 @@ struct PrepScratch {
 
 /* The global loads of one item's preparation, issued ahead of their use
  * and kept as loaded (no arithmetic on them until prep_compute, so that
- * none of them waits for another): lane t < 60 its own value (iw, i) =
- * (t / 10, t % 10), D^-1 entries t and 64 + t, and for t < 36 the trace
- * value (field t / 6, face node t % 6) of each ghost or lagged inflow
- * face. */
-template <typename R> struct PrepLoads {
-	R fv, u[4];
+ * none of them waits for another): lane t < 60 its value (iw, i) = (t / 10,
+ * t % 10) of the item's velocity (fr[0]) or, when the state still has to
+ * be relaxed, of every velocity (fr[0..NV-1]); D^-1 entries t and 64 + t.
+ * The traces of ghost and lagged inflow faces (boundary items and cycles
+ * only) are read by prep_compute itself. */
+template <typename R, int NV> struct PrepLoads {
+	R fr[NV];
 	double d0, d1;
 };
 
@@ -324,32 +383,41 @@ __device__ inline double uni(double x)
 	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <typename R>
+template <typename R, int NV>
 __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict__ fo,
-				  const double *__restrict__ dinv, int ne, int neq, int nv,
-				  PrepLoads<R> &P)
+				  const double *__restrict__ dinv, int ne, int relax_in,
+				  PrepLoads<R, NV> &P)
 {
 	const int t = lane_id();
-	P.fv = t < 60 ? fo[((long)d.v * ne + d.e) * 60 + t] : (R)0;
+	if (relax_in) {
+#pragma unroll
+		for (int u = 0; u < NV; u++)
+			P.fr[u] = t < 60 ? fo[((long)u * ne + d.e) * 60 + t] : (R)0;
+	} else {
+		P.fr[0] = t < 60 ? fo[((long)d.v * ne + d.e) * 60 + t] : (R)0;
+#pragma unroll
+		for (int u = 1; u < NV; u++)
+			P.fr[u] = (R)0;
+	}
 	const double *D = dinv + 100L * it;
 	P.d0 = D[t];
 	P.d1 = t < 36 ? D[64 + t] : 0.0;
-	const int gf = given_faces(d);
-#pragma unroll
-	for (int f = 0; f < 4; f++)
-		P.u[f] = (gf & (1 << f)) && t < 36 ? fo[trace_index(d, f, t, ne, neq, nv)] : (R)0;
 }
 
 /* u = the ghost value, or on a lagged face (1 - theta) f_nb + theta x_nb
  * with x_nb from xl (none when xl is null: both halves of a lagged trace
- * are given values). Fills B (G_s and the slots) and returns the lane's
- * base value (t < 60). */
-template <typename R>
-__device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
-				      const R *__restrict__ xl, int ne, int neq, int nv,
-				      const SweepTables &T, ItemBuf &B, PrepScratch &W)
+ * are given values). With relax_in the item's values are relaxed first (the
+ * state holds the previous transport's output) and rounded to R as a stored
+ * state would be. Fills B (G_s and the slots) and fv (the lane's value of
+ * the item, t < 60) and returns the lane's base value. */
+template <typename R, int NV>
+__device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> &P,
+				      const R *__restrict__ fo, const R *__restrict__ xl,
+				      int ne, int neq, int nv, int relax_in, const KinPar &kp,
+				      const SweepTables &T, ItemBuf &B, PrepScratch &W, R &fv)
 {
 	const int t = threadIdx.x;
+	fv = relax_in ? (R)relax_value<NV>(kp, P.fr, d.v) : P.fr[0];
 	const int gf = given_faces(d), sf = slot_faces(d);
 	const int iw = t / 10, i = t % 10;
 	W.D[t / 10][t % 10] = P.d0;
@@ -374,11 +442,12 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
 		if (!(gf & (1 << f)))
 			continue;
 		if (t < 36) {
-			double u = (double)P.u[f];
+			const long g = trace_index(d, f, lane_id(), ne, neq, nv);
+			double u = (double)fo[g];
 			if (!(d.flags & (16 << f))) {
 				u = (1.0 - THETA) * u;
 				if (xl)
-					u += THETA * (double)xl[trace_index(d, f, t, ne, neq, nv)];
+					u += THETA * (double)xl[g];
 			}
 			W.S[t] = u;
 		}
@@ -393,7 +462,7 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
 		__syncthreads();
 	}
 	if (t < 60)
-		W.S[t] = 2.0 * (double)P.fv - y; /* Z[iw][i] */
+		W.S[t] = 2.0 * (double)fv - y; /* Z[iw][i] */
 	__syncthreads();
 	double base = 0.0;
 	if (t < 60) {
@@ -401,7 +470,7 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R> &P,
 #pragma unroll
 		for (int l = 0; l < 10; l++)
 			s += W.D[i][l] * W.S[10 * iw + l];
-		base = s - (double)P.fv;
+		base = s - (double)fv;
 		/* G_s, lane t = 6 r + b: row r, face node b */
 		const int r = t / 6, b = t % 6;
 		int slot = 0;
@@ -583,12 +652,12 @@ __device__ inline void load_tables(SweepTables &T)
 /* One upwind level per launch, one item per 64-lane block; the inflow
  * traces are the hand-off words of earlier levels (a word without the
  * sweep's epoch is reported in *status). */
-template <typename R>
+template <typename R, int NV>
 __global__ __launch_bounds__(64) void k_sweep_level(
 	const ItemDesc *__restrict__ desc, int begin, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
 	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
-	const unsigned *__restrict__ epoch_dev, int *status)
+	const unsigned *__restrict__ epoch_dev, int *status, int relax_in, KinPar kp)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B;
@@ -598,16 +667,17 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
 	const int it = begin + blockIdx.x;
 	const ItemDesc d = desc[it];
-	PrepLoads<R> P;
-	prep_issue(it, d, fo, dinv, ne, neq, nv, P);
-	const double base = prep_compute(d, P, xl, ne, neq, nv, T, B, W);
+	PrepLoads<R, NV> P;
+	prep_issue(it, d, fo, dinv, ne, relax_in, P);
+	R fv;
+	const double base = prep_compute(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B, W, fv);
 	const int t = threadIdx.x;
 	const unsigned pending =
 		ll_poll<R>(B, ll, ne, epoch, t < 36 ? (1u << B.nslot) - 1u : 0u, X);
 	if (__ballot(pending != 0) && t == 0)
 		__hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	__syncthreads();
-	sweep_finish<R>(B, X, P.fv, base, fn, ll, ne, epoch);
+	sweep_finish<R>(B, X, fv, base, fn, ll, ne, epoch);
 }
 
 /* an ItemDesc held one dword per lane (lane k: dword k), made uniform */
@@ -638,12 +708,13 @@ __device__ inline ItemDesc desc_from_lanes(int x)
  * records a 100 us busy interval, and the largest number of overlapping
  * intervals is a set of blocks that were resident together.
  */
-template <typename R, bool FULL_POLL>
+template <typename R, bool FULL_POLL, int NV>
 __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
 	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
-	const unsigned *__restrict__ epoch_dev, int *status, u64 *disc)
+	const unsigned *__restrict__ epoch_dev, int *status, u64 *disc, int relax_in,
+	KinPar kp)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
@@ -672,10 +743,9 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 	double base;
 	{
 		const ItemDesc d = desc[it];
-		PrepLoads<R> P;
-		prep_issue(it, d, fo, dinv, ne, neq, nv, P);
-		base = prep_compute(d, P, xl, ne, neq, nv, T, B[0], W);
-		fv = P.fv;
+		PrepLoads<R, NV> P;
+		prep_issue(it, d, fo, dinv, ne, relax_in, P);
+		base = prep_compute(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W, fv);
 	}
 	ItemDesc dn; /* descriptor of item it + G */
 	if (it + G < nitems)
@@ -683,21 +753,23 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 	for (; it < nitems; it += G, k ^= 1) {
 		const int next = it + G;
 		const bool has_next = next < nitems;
-		PrepLoads<R> P;
+		PrepLoads<R, NV> P;
 		if (has_next)
-			prep_issue(next, dn, fo, dinv, ne, neq, nv, P);
+			prep_issue(next, dn, fo, dinv, ne, relax_in, P);
 		/* descriptor of item next + G: one dword per lane (t < 20) */
 		int dnn = 0;
 		const int tl = lane_id();
 		if (next + G < nitems && tl < (int)(sizeof(ItemDesc) / 4))
 			dnn = reinterpret_cast<const int *>(desc + next + G)[tl];
 		double nbase = 0.0;
+		R nfv = (R)0;
 		unsigned pending = t < 36 ? (1u << uni(B[k].nslot)) - 1u : 0u;
 		pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 		if (__ballot(pending != 0)) {
 			/* prepare the next item, then watch one word per slot */
 			if (has_next)
-				nbase = prep_compute(dn, P, xl, ne, neq, nv, T, B[k ^ 1], W);
+				nbase = prep_compute(dn, P, fo, xl, ne, neq, nv, relax_in, kp, T,
+						     B[k ^ 1], W, nfv);
 			int spins = 0;
 			if (!FULL_POLL) {
 				bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
@@ -720,10 +792,11 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 			__syncthreads(); /* X complete */
 			sweep_finish<R>(B[k], X, fv, base, fn, ll, ne, epoch);
 			if (has_next)
-				nbase = prep_compute(dn, P, xl, ne, neq, nv, T, B[k ^ 1], W);
+				nbase = prep_compute(dn, P, fo, xl, ne, neq, nv, relax_in, kp, T,
+						     B[k ^ 1], W, nfv);
 		}
 		if (has_next) {
-			fv = P.fv;
+			fv = nfv;
 			base = nbase;
 		}
 		if (next + G < nitems)
@@ -737,18 +810,34 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
  * reports 20 blocks per CU for 8000 B of LDS where 18 fit), and a
  * cooperative launch sized by it does not have all its blocks resident.
  */
+template <typename R>
+using SweepKernel = void (*)(const ItemDesc *, int, const R *, R *, const R *, const double *,
+			     int, int, int, u64 *, unsigned, const unsigned *, int *, u64 *, int,
+			     KinPar);
+
+/* the persistent sweep of this context: precision, way of polling and
+ * number of velocities */
+template <typename R> static SweepKernel<R> sweep_kernel(const gdh_ctx *c)
+{
+	if (c->kp.nv == 4)
+		return c->sweep_poll ? k_sweep_persistent<R, true, 4> : k_sweep_persistent<R, false, 4>;
+	if (c->kp.nv == 6)
+		return c->sweep_poll ? k_sweep_persistent<R, true, 6> : k_sweep_persistent<R, false, 6>;
+	fprintf(stderr, "gdon3d-hip: no transport sweep for %d velocities\n", c->kp.nv);
+	abort();
+}
+
 template <typename R> static int sweep_residency(gdh_ctx *c)
 {
 	int dev = 0, ncu = 0, per_cu = 0;
 	HIPCHECK(hipGetDevice(&dev));
 	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<R, false>,
-							      64, 0));
+	const SweepKernel<R> kern = sweep_kernel<R>(c);
+	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
-	k_sweep_persistent<R, false><<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr,
-							nullptr, nullptr, 0, 0, 0, nullptr, 0,
-							nullptr, nullptr, disc);
+	kern<<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
+				       nullptr, 0, nullptr, nullptr, disc, 0, c->kp);
 	HIPCHECK(hipGetLastError());
 	std::vector<u64> h(2 * (size_t)g0);
 	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(),
@@ -788,7 +877,6 @@ void sweep_setup(gdh_ctx *c)
 	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	c->epoch = 0;
-	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 	const char *mode = getenv("GDN_SWEEP");
 	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
 	/* waiting items poll one word per inflow face, then the traces
@@ -798,6 +886,7 @@ void sweep_setup(gdh_ctx *c)
 	 * (binary32). GDN_SWEEP_POLL=sentinel|full overrides. */
 	const char *poll = getenv("GDN_SWEEP_POLL");
 	c->sweep_poll = poll ? !strcmp(poll, "full") : c->fp32;
+	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 }
 
 __global__ void k_epoch_bump(unsigned *e)
@@ -806,7 +895,7 @@ __global__ void k_epoch_bump(unsigned *e)
 }
 
 template <typename R>
-static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
+static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 {
 	/* the sweep's epoch: on the host, or (batches, captured in graphs) in
 	 * device memory advanced by a one-thread node */
@@ -826,14 +915,21 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 	int *status = c->status;
 	unsigned epoch = c->epoch;
 	const unsigned *edev = c->epoch_dev;
+	KinPar kp = c->kp;
 	if (c->sweep_mode == 0) {
 		for (int L = 0; L < c->nlevels; L++) {
 			const int b = c->level_ptr[L], n = c->level_ptr[L + 1] - b;
 			if (!n)
 				continue;
 			c->prof_launches++;
-			k_sweep_level<R><<<n, 64, 0, c->stream>>>(desc, b, fo, fn, xl, dinv, nv, ne,
-								  neq, ll, epoch, edev, status);
+			if (nv == 4)
+				k_sweep_level<R, 4><<<n, 64, 0, c->stream>>>(
+					desc, b, fo, fn, xl, dinv, nv, ne, neq, ll, epoch, edev, status,
+					relax_in, kp);
+			else
+				k_sweep_level<R, 6><<<n, 64, 0, c->stream>>>(
+					desc, b, fo, fn, xl, dinv, nv, ne, neq, ll, epoch, edev, status,
+					relax_in, kp);
 		}
 		HIPCHECK(hipGetLastError());
 		return;
@@ -841,27 +937,27 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl)
 	c->prof_launches++;
 	const R *fo_c = fo;
 	const int grid = std::min(c->sweep_grid, nitems);
-	auto kern = c->sweep_poll ? k_sweep_persistent<R, true> : k_sweep_persistent<R, false>;
+	const SweepKernel<R> kern = sweep_kernel<R>(c);
 	if (edev) {
 		/* an ordinary launch with the measured resident grid, since a
 		 * captured stream takes no cooperative launch */
 		kern<<<grid, 64, 0, c->stream>>>(desc, nitems, fo_c, fn, xl, dinv, nv, ne, neq, ll,
-						 0u, edev, status, disc);
+						 0u, edev, status, disc, relax_in, kp);
 		HIPCHECK(hipGetLastError());
 		return;
 	}
 	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
-			 &epoch, &edev, &status, &disc };
+			 &epoch, &edev, &status, &disc, &relax_in, &kp };
 	HIPCHECK(hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(64), args, 0,
 					    c->stream));
 }
 
-void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl)
+void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl, int relax_in)
 {
 	if (c->fp32)
-		sweep_step_t<float>(c, (float *)fo, (float *)fn, (const float *)xl);
+		sweep_step_t<float>(c, (float *)fo, (float *)fn, (const float *)xl, relax_in);
 	else
-		sweep_step_t<double>(c, (double *)fo, (double *)fn, (const double *)xl);
+		sweep_step_t<double>(c, (double *)fo, (double *)fn, (const double *)xl, relax_in);
 }
 
 /* f <- omega f_eq(w(f)) + (1 - omega) f at DoFs [ipg0, n)
@@ -896,21 +992,6 @@ __global__ __launch_bounds__(256) void k_relax_dof(R *__restrict__ f, KinPar kp,
 		for (int v = 0; v < NV; v++)
 			f[kin_at(ne, neq, NV, ipg, iw, v)] =
 				(R)(om * feq_comp(kp, w, q, iw, v) + (1. - om) * fo[iw][v]);
-}
-
-/* row iw of maxwell_qM(w), by selects (iw differs between lanes) */
-__device__ inline void maxwell_q_row(const double w[6], int iw, double r[3])
-{
-	double q[6][3];
-	maxwell_qM(w, q);
-	r[0] = r[1] = r[2] = 0.0;
-#pragma unroll
-	for (int k = 0; k < 6; k++)
-		if (iw == k) {
-			r[0] = q[k][0];
-			r[1] = q[k][1];
-			r[2] = q[k][2];
-		}
 }
 
 /*
@@ -1197,7 +1278,7 @@ static void cap_setup(gdh_ctx *c)
 			k_cap_unit<double><<<nblocks(6 * nv, 64), 64, 0, c->stream>>>(
 				(double *)c->wz, c->cap_ipg, c->cap_off, bt, nv, c->ne, c->neq);
 		HIPCHECK(hipGetLastError());
-		sweep_step(c, zero, out, c->wz);
+		sweep_step(c, zero, out, c->wz, 0);
 		cap_gather(c, out);
 		HIPCHECK(hipMemcpyAsync(th.data(), c->cap_t, sizeof(double) * th.size(),
 					hipMemcpyDeviceToHost, c->stream));
@@ -1251,35 +1332,107 @@ static void cap_solve(gdh_ctx *c)
 	HIPCHECK(hipGetLastError());
 }
 
+/*
+ * One step of the kinetic scheme: transport, then the node-local relaxation
+ * (src/dg/relaxation.c:130-160). The relaxation is applied by the next
+ * step's sweep as it reads its input (each item relaxes its own values
+ * from those of every velocity at its nodes; consumers see the published
+ * averaged traces, never a neighbour's stored values); after a step the
+ * state holds the transport's output and relax_pending is set.
+ * relax_materialize applies it in place wherever the state itself is needed
+ * (download, L2 error, relaxation of every DoF, and before the sweeps over
+ * upwind cycles, whose lagged faces read neighbours' stored values).
+ */
 extern "C" void gdh_relax_step(gdh_ctx *c)
 {
+	/* unless fused (see relax_choose), and over upwind cycles (lagged
+	 * faces read the neighbour's stored values), the relaxation of the last
+	 * step is applied before the sweep */
+	if (!c->relax_fused)
+		relax_materialize(c);
 	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
+	const int rin = c->relax_pending;
 	/* Ghost DoFs are identity rows of M1 and M2; the sweep reads them from
 	 * fo and nothing reads the ghost region of fn before the next ghost
 	 * update overwrites it. */
 	prof_begin(c, 0);
 	if (!c->nlagged) {
-		sweep_step(c, fo, fn, nullptr);
+		sweep_step(c, fo, fn, nullptr, rin);
 	} else {
 		/* exact over the upwind cycles: two sweeps and a small solve */
-		sweep_step(c, fo, fn, nullptr);
+		sweep_step(c, fo, fn, nullptr, rin);
 		cap_gather(c, fn);
 		cap_solve(c);
-		sweep_step(c, fo, fn, c->wz);
+		sweep_step(c, fo, fn, c->wz, rin);
 		c->lag_last = 2;
 		c->lag_max = 2;
 	}
 	prof_end(c, 0);
-	const long n = 10L * c->ne;
-	prof_begin(c, 1);
-	relax_launch(c, fn, n);
-	prof_end(c, 1);
+	c->relax_pending = 1;
 	c->cur = 1 - c->cur;
+	if (!c->relax_fused)
+		relax_materialize(c);
+}
+
+/*
+ * Fused or separate relaxation, by measurement on this mesh: the fused
+ * form saves the relaxation kernel's pass over the state but gives every
+ * sweep item the loads of all velocities at its nodes and the relaxation
+ * arithmetic. Where the sweep is bound by its dependency chain that work
+ * is hidden, where it is bound by its item rate it is not (measured on
+ * MI355X, D3Q4 FP64: 1.00 vs 1.03 ms per step at 32^3, 7.06 vs 6.57 ms at
+ * 64^3). Both forms are timed (five sweeps each, after one) from the
+ * current state into the other buffer; nothing of the state changes.
+ */
+static void relax_choose(gdh_ctx *c)
+{
+	c->relax_fused = 0;
+	const char *env = getenv("GDN_RELAX_FUSED");
+	if (c->nlagged || (env && !strcmp(env, "0")))
+		return;
+	if (env && !strcmp(env, "1")) {
+		c->relax_fused = 1;
+		return;
+	}
+	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
+	const int prof = c->prof;
+	c->prof = 0;
+	float ms[2];
+	for (int fused = 0; fused < 2; fused++) {
+		for (int r = 0; r < 6; r++) {
+			if (r == 1)
+				HIPCHECK(hipEventRecord(c->ev0, c->stream));
+			sweep_step(c, fo, fn, nullptr, fused);
+			if (!fused)
+				relax_launch(c, fn, 10L * c->ne);
+		}
+		HIPCHECK(hipEventRecord(c->ev1, c->stream));
+		HIPCHECK(hipEventSynchronize(c->ev1));
+		HIPCHECK(hipEventElapsedTime(&ms[fused], c->ev0, c->ev1));
+	}
+	c->prof = prof;
+	c->relax_fused = ms[1] < ms[0];
+}
+
+extern "C" int gdh_relax_fused(gdh_ctx *c)
+{
+	return c->relax_fused;
+}
+
+void relax_materialize(gdh_ctx *c)
+{
+	if (!c->scheme || !c->relax_pending)
+		return;
+	prof_begin(c, 1);
+	relax_launch(c, c->w[c->cur], 10L * c->ne);
+	prof_end(c, 1);
+	c->relax_pending = 0;
 }
 
 /* relaxation of every DoF, ghosts included (src/dg/dg.c:109-147) */
 extern "C" void gdh_apply_relaxation(gdh_ctx *c)
 {
+	relax_materialize(c);
 	const long n = c->neq;
 	relax_launch(c, c->w[c->cur], n);
 }

@@ -174,6 +174,7 @@ def load_library(path=LIB_PATH):
         "gdh_mlp_predict": (None, [C.c_void_p, c_float_p, C.c_int, c_float_p]),
         "gdh_learned_data": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_float_p, c_float_p]),
         "gdh_sweep_grid": (C.c_int, [C.c_void_p]),
+        "gdh_relax_fused": (C.c_int, [C.c_void_p]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
@@ -467,8 +468,18 @@ class BlockNet:
 
 def sweep_grid(stepper):
     """Workgroups of the one-launch transport sweep (measured residency)."""
+    if not stepper.h:
+        raise RuntimeError("stepper already freed")
     lib = load_library()
     return lib.gdh_sweep_grid(lib.dg_stepper_device_context(stepper.h))
+
+
+def relax_fused(stepper):
+    """1 if the sweep applies each step's relaxation as it reads its input."""
+    if not stepper.h:
+        raise RuntimeError("stepper already freed")
+    lib = load_library()
+    return lib.gdh_relax_fused(lib.dg_stepper_device_context(stepper.h))
 
 
 def learned_data(stepper, idx):

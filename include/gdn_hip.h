@@ -171,6 +171,12 @@ void gdh_transport_stats(gdh_ctx *ctx, int out[3]);
 /* Workgroups of the one-launch transport sweep: the number measured to be
  * resident together at set-up (no counterpart in the reference). */
 int gdh_sweep_grid(gdh_ctx *ctx);
+/* 1 when the relaxation of each step is applied by the next step's sweep as
+ * it reads its input, 0 when it runs as a pass of its own after the sweep:
+ * chosen at gdh_relax_setup by timing both on the mesh (GDN_RELAX_FUSED=0|1
+ * forces one); the results agree to rounding. No counterpart in the
+ * reference. */
+int gdh_relax_fused(gdh_ctx *ctx);
 
 /* ---- batches (no counterpart in the reference: one drop-in simulation
  * per call there). The context's mesh is the disjoint union of ninst

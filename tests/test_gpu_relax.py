@@ -85,3 +85,43 @@ def test_persistent_sweep_equals_level_sweep(n, jitter, tmax, poll, monkeypatch)
         out[mode] = sim.wn()
         sim.free()
     assert np.array_equal(out["level"], out["persistent"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kin,precision", [("d3q4", "fp64"), ("d3q4", "fp32"), ("d3q6", "fp64")])
+def test_fused_relaxation_equals_separate(kin, precision, monkeypatch):
+    """The relaxation applied by the next sweep as it reads its input
+    (GDN_RELAX_FUSED=1) and as a pass of its own after each sweep
+    (GDN_RELAX_FUSED=0) give the same solve: both round the relaxed values
+    to the state precision, and the arithmetic differs only in how the
+    compiler contracts it (relative L2 <= 1e-13 FP64, <= 1e-5 FP32); the
+    fused form also through several steps, the L2 error, the download and a
+    second advance."""
+    path = kuhn_msh(6, 0.1, 41)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    out = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+        mesh = G.Mesh(path=path)
+        sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic=kin))
+        sim.set_time(0.4, 0.0, 5.0)
+        st = G.Stepper(sim, precision=precision)
+        assert G.relax_fused(st) == int(fused)
+        half = sim.s.itermax // 2
+        st.advance(half)
+        err_mid = st.error_l2()
+        st.finish()
+        mid = sim.wn()
+        st.advance(sim.s.itermax - half)
+        st.sync()
+        err = st.error_l2()
+        st.finish()
+        out[fused] = (mid, sim.wn(), err_mid, err)
+        st.free()
+        sim.free()
+    tol = 1e-13 if precision == "fp64" else 1e-5
+    a, b = out["0"], out["1"]
+    print(f"fused vs separate relaxation {kin} {precision}: mid {rel_l2(b[0], a[0]):.3e} "
+          f"end {rel_l2(b[1], a[1]):.3e}", flush=True)
+    assert rel_l2(b[0], a[0]) <= tol and rel_l2(b[1], a[1]) <= tol
+    assert abs(b[2] - a[2]) <= tol * a[2] and abs(b[3] - a[3]) <= tol * a[3]
