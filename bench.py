@@ -40,7 +40,10 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # bytes, WRITE_SIZE all of them); per element of the workloads measured there
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
-SWEEP_TRAFFIC_BYTES_PER_ELEM = (1417840.5 / 0.5 + 1108374.3) * 1024 / 196608
+SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # relaxation fused into the sweep (1) or separate (0)
+    1: (2077260.8 / 0.5 + 1108102.5) * 1024 / 196608,
+    0: (1345019.2 / 0.5 + 1108102.5) * 1024 / 196608,
+}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (1493823.2 / 0.5 + 1474560.0) * 1024 / 1572864
 
 
@@ -467,8 +470,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem if default_workload else None),
-            "traffic_gb_s": (SWEEP_TRAFFIC_BYTES_PER_ELEM * nelem / sweep_s / 1e9
+            "traffic": (SWEEP_TRAFFIC_BYTES_PER_ELEM[relax_fused] * nelem
+                        if default_workload else None),
+            "traffic_gb_s": (SWEEP_TRAFFIC_BYTES_PER_ELEM[relax_fused] * nelem / sweep_s / 1e9
                              if default_workload else None),
             "kernel": "k_sweep_persistent (one launch = the whole sweep of one step)",
             "bytes_per_element_step": sweep_bytes,
