@@ -86,9 +86,13 @@ def run_batch(cases, kinetic, cfl, steps, precision, split=None):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fused", ["0", "1"])
 @pytest.mark.parametrize("kinetic,cases,cfl", [(None, CASES_RK3, 0.5), ("d3q4", CASES_KIN, 5.0)])
-def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl):
-    print(f"batch {kinetic or 'rk3'}", flush=True)
+def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, monkeypatch):
+    """Bit for bit, with the relaxation form fixed (separate, or fused into
+    the sweep: GDN_RELAX_FUSED) for the batch and the single instances."""
+    monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+    print(f"batch {kinetic or 'rk3'} fused {fused}", flush=True)
     steps = 7
     out, tiles = run_batch(cases, kinetic, cfl, steps, "fp64")
     print(f"  tiles {tiles}", flush=True)
@@ -98,9 +102,17 @@ def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl):
 
 
 @pytest.mark.gpu
-def test_batch_graph_replay_split_steps():
-    a, _ = run_batch(CASES_RK3, None, 0.5, 7, "fp64")
-    b, _ = run_batch(CASES_RK3, None, 0.5, 7, "fp64", split=[3, 1, 2, 1])
+@pytest.mark.parametrize("kinetic,cases,cfl,fused",
+                         [(None, CASES_RK3, 0.5, "0"), ("d3q4", CASES_KIN, 5.0, "0"),
+                          ("d3q4", CASES_KIN, 5.0, "1")])
+def test_batch_graph_replay_split_steps(kinetic, cases, cfl, fused, monkeypatch):
+    """7 steps in one call or as 3 + 1 + 2 + 1 (graph pairs of both parities,
+    direct odd steps; with the fused relaxation the first step of the batch
+    is taken directly so that every captured sweep relaxes its input) give
+    the same states bit for bit."""
+    monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+    a, _ = run_batch(cases, kinetic, cfl, 7, "fp64")
+    b, _ = run_batch(cases, kinetic, cfl, 7, "fp64", split=[3, 1, 2, 1])
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
 
