@@ -44,7 +44,10 @@ SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # relaxation fused into the sweep (1) or separ
     1: (2077260.8 / 0.5 + 1108102.5) * 1024 / 196608,
     0: (1345019.2 / 0.5 + 1108102.5) * 1024 / 196608,
 }
-RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = (1493823.2 / 0.5 + 1474560.0) * 1024 / 1572864
+RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
+    ("fp64", "k_rk3_pair"): (1317320.8 / 0.5 + 1474560.0) * 1024 / 1572864,
+    ("fp32", "k_rk3_tile"): (837410.6 / 0.5 + 737280.0) * 1024 / 1572864,
+}
 
 
 def cpu_baseline(n, steps):
@@ -106,6 +109,10 @@ def bench_rk3(args):
     stage_s = prof[0] / (3 * args.profile_steps)
     s_val = 4 if args.precision == "fp32" else 8
     bytes_elem = 4 * 60 * s_val + 112
+    # the stage kernel the library runs: two lanes per element (binary64
+    # default), one (binary32 default), or GDN_RK3's choice
+    kernel = {"soa": "k_rk3_soa", "pair": "k_rk3_pair", "tile": "k_rk3_tile"}.get(
+        os.environ.get("GDN_RK3", ""), "k_rk3_tile" if args.precision == "fp32" else "k_rk3_pair")
     achieved = bytes_elem * nelem / stage_s / 1e9
     print(f"[bench rk3] n={n} elements={nelem} dt={sim.s.dt:.4e} {args.steps} steps "
           f"{elapsed:.3f}s stage kernel {stage_s*1e3:.3f} ms ghosts "
@@ -125,11 +132,10 @@ def bench_rk3(args):
                    "l2_error": err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (RK3_TRAFFIC_BYTES_PER_ELEM_STAGE * nelem
-                                 if args.precision == "fp64" and n == 64
-                                 and os.environ.get("GDN_RK3") != "soa" else None),
-                     "kernel": ("k_rk3_soa" if os.environ.get("GDN_RK3") == "soa"
-                                else "k_rk3_tile"),
+                     "traffic": (RK3_TRAFFIC_BYTES_PER_ELEM_STAGE[(args.precision, kernel)] * nelem
+                                 if n == 64 and (args.precision, kernel)
+                                 in RK3_TRAFFIC_BYTES_PER_ELEM_STAGE else None),
+                     "kernel": kernel,
                      "bytes_per_element_stage": bytes_elem,
                      "avg_launch_us": stage_s * 1e6},
     }

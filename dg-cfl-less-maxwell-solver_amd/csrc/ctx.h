@@ -98,7 +98,8 @@ struct gdh_ctx {
 	std::vector<int> bt_e0, bt_nb0; /* first element / boundary face per instance */
 	hipGraphExec_t graph[2];
 	int rk3_bs, relax_bs; /* tile sizes (autotuned for batches) */
-	int rk3_tile; /* 1: RK3 stage with the tile state staged in LDS */
+	int rk3_tile; /* RK3 stage: 0 global state, 1 LDS tile, 2 LDS tile with
+		       * two lanes per element */
 	long nlagged; /* inflow faces lagged to break upwind cycles */
 	void *wx; /* zero state (set-up of the capacitance matrices) */
 	void *wz; /* lagged values z, scattered into a state-shaped buffer */

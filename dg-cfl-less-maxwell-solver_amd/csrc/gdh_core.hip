@@ -225,11 +225,19 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->graph[0] = c->graph[1] = nullptr;
 	c->rk3_bs = 128;
 	{
-		/* RK3 stage kernel: the LDS-tiled one (default), or one element per
-		 * thread over the global state (GDN_RK3=soa; both compute the same
-		 * arithmetic in the same order) */
+		/* RK3 stage kernel, all three over the same arithmetic: the tile's
+		 * state staged in LDS with two lanes per element ("pair", the binary64
+		 * default: 1.17 vs 1.34 ms per stage at 64^3) or one ("tile", the
+		 * binary32 default: 0.68 vs 0.71 ms), or one element per thread over
+		 * the global state ("soa"); GDN_RK3=pair|tile|soa overrides */
 		const char *k = getenv("GDN_RK3");
-		c->rk3_tile = !(k && !strcmp(k, "soa"));
+		c->rk3_tile = c->fp32 ? 1 : 2;
+		if (k && !strcmp(k, "soa"))
+			c->rk3_tile = 0;
+		else if (k && !strcmp(k, "tile"))
+			c->rk3_tile = 1;
+		else if (k && !strcmp(k, "pair"))
+			c->rk3_tile = 2;
 	}
 	c->relax_bs = 256;
 	c->cap_qtot = 0;

@@ -172,15 +172,20 @@ __device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
 	const T overr = (T)(1.0 / (sqrt(vnd[0] * vnd[0] + vnd[1] * vnd[1] +
 					vnd[2] * vnd[2]) + 1e-16));
 	const T vn[3] = { (T)vnd[0], (T)vnd[1], (T)vnd[2] };
-	/* the whole neighbour trace first, one load path for every lane (a
-	 * neighbour inside the tile was just read by this block and comes from
-	 * the cache): one round trip per face */
-	(void)c0;
+	/* the whole neighbour trace first, one round trip per face: in binary32
+	 * from LDS for a neighbour inside the tile (64^3: 0.68 vs 0.70 ms per
+	 * stage), else, and always in binary64 (1.34 vs 1.32 ms), from global
+	 * memory */
+	const unsigned nt = (unsigned)(nb - c0);
 	T wra[6][6];
 #pragma unroll
 	for (int b = 0; b < 6; b++) {
 		const int j = (map >> (4 * b)) & 15;
-		if (nb >= 0) {
+		if (sizeof(T) == 4 && nt < 64u) {
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[b][q] = Ws[(q * 10 + j) * 64 + nt];
+		} else if (nb >= 0) {
 #pragma unroll
 			for (int q = 0; q < 6; q++)
 				wra[b][q] = W[(q * 10 + j) * ne + nb];
@@ -371,6 +376,223 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 		}
 }
 
+template <typename T> __device__ __forceinline__ void pin3(T (&acc)[3][10])
+{
+#pragma unroll
+	for (int p = 0; p < 3; p++)
+#pragma unroll
+		for (int m = 0; m < 10; m++)
+			asm volatile("" : "+v"(acc[p][m]));
+	__builtin_amdgcn_sched_barrier(0);
+}
+
+/*
+ * Two lanes per element: a 64-lane block takes 32 consecutive storage
+ * columns (15 KB of LDS in FP64, twice the blocks per CU of k_rk3_tile);
+ * lane half h = 0 forms the E rows of the update, h = 1 the H rows. The
+ * code is the same for both halves: the curl terms of the H rows are those
+ * of the E rows with E and H exchanged and the sign reversed (A(n): E-flux
+ * = -n x H, H-flux = n x E), so a half differs only by its source fields
+ * (H or E) and a sign sigma = -1 or +1. Both halves form the face fluxes;
+ * each lifts its three components.
+ */
+template <int J, typename T>
+__device__ __forceinline__ void volume_pair(T (&acc)[3][10], const T *Ws, int c, int qs0,
+					    T sigma, const double (&ct)[9],
+					    const Rk3Tables<T> *__restrict__ Tb)
+{
+	T wq[10];
+#pragma unroll
+	for (int j = 0; j < 10; j++)
+		wq[j] = Ws[((qs0 + J) * 10 + j) * 32 + c];
+	/* source component J feeds outputs o1, o2 with (sign * n_comp) */
+	constexpr int o1 = J == 0 ? 1 : 0, o2 = J == 2 ? 1 : 2;
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		const T nx = (T)ct[0 + k], ny = (T)ct[3 + k], nz = (T)ct[6 + k];
+		/* J = 0: o1 -= s nz, o2 += s ny; J = 1: o0 += s nz, o2 -= s nx;
+		 * J = 2: o0 -= s ny, o1 += s nx */
+		const T c1 = J == 0 ? sigma * nz : J == 1 ? -sigma * nz : sigma * ny;
+		const T c2 = J == 0 ? -sigma * ny : J == 1 ? sigma * nx : -sigma * nx;
+#pragma unroll
+		for (int m = 0; m < 10; m++) {
+			T hm = T(0);
+#pragma unroll
+			for (int j = 0; j < 10; j++)
+				hm += Tb->minv_int[k][m][j] * wq[j];
+			acc[o1][m] -= c1 * hm;
+			acc[o2][m] -= c2 * hm;
+		}
+		pin3(acc);
+	}
+}
+
+/* exchange between lane halves: lanes 32-63 of a take lanes 0-31 of b and
+ * the other way round (v_permlane32_swap), one dword at a time */
+__device__ __forceinline__ void swap_halves(float &a, float &b)
+{
+	auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a),
+						  __builtin_bit_cast(unsigned, b), false, false);
+	a = __builtin_bit_cast(float, (unsigned)r[0]);
+	b = __builtin_bit_cast(float, (unsigned)r[1]);
+}
+
+__device__ __forceinline__ void swap_halves(double &a, double &b)
+{
+	const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+	const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+	auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+	auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32),
+						   false, false);
+	a = __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi[0] << 32) | (unsigned)lo[0]);
+	b = __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi[1] << 32) | (unsigned)lo[1]);
+}
+
+/* Neighbour trace of face F for the pair layout: lane half h gathers the
+ * six fields at face nodes b = 3 h + b' (b' < 3), from the neighbour's
+ * columns or the ghost DoFs of boundary face -1 - nb */
+template <int F, typename T>
+__device__ __forceinline__ void gather_pair(T (&wra)[3][6], const T *Ws, const T *__restrict__ W,
+					    const T *__restrict__ GH, long ne, long ngh, int h,
+					    int c0, int nb, int map)
+{
+	const unsigned nt = (unsigned)(nb - c0);
+#pragma unroll
+	for (int bp = 0; bp < 3; bp++) {
+		const int b = 3 * h + bp;
+		if (nt < 32u) {
+			const int j = (map >> (4 * b)) & 15;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[bp][q] = Ws[(q * 10 + j) * 32 + nt];
+		} else if (nb >= 0) {
+			const int j = (map >> (4 * b)) & 15;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[bp][q] = W[(q * 10 + j) * ne + nb];
+		} else {
+			const long k = 6L * (-1 - nb) + b;
+#pragma unroll
+			for (int q = 0; q < 6; q++)
+				wra[bp][q] = GH[q * ngh + k];
+		}
+	}
+}
+
+/* Face F's flux and lift for the pair layout, from the gathered trace
+ * wra: each half forms the full six-component flux at its three nodes;
+ * one swap per component then gives every lane, for its three components,
+ * the flux at node b' (fl[p]) and node 3 + b' (fl[3 + p]): half 0 keeps
+ * its E values of nodes 0-2 and takes its partner's of nodes 3-5, half 1
+ * takes its partner's H values of nodes 0-2 and keeps its own of nodes
+ * 3-5. */
+template <int F, typename T>
+__device__ __forceinline__ void face_pair(T (&acc)[3][10], const T *Ws, const T (&wra)[3][6],
+					  int c, int h, int nb, const double (&ctd)[9], int bflux,
+					  const Rk3Tables<T> *__restrict__ Tb)
+{
+	double vnd[3];
+	face_normal(ctd, F, vnd);
+	const T overr = (T)(1.0 / (sqrt(vnd[0] * vnd[0] + vnd[1] * vnd[1] +
+					vnd[2] * vnd[2]) + 1e-16));
+	const T vn[3] = { (T)vnd[0], (T)vnd[1], (T)vnd[2] };
+#pragma unroll
+	for (int bp = 0; bp < 3; bp++) {
+		const int i = h ? face_node(F, 3 + bp) : face_node(F, bp);
+		T wl[6], fl[6];
+#pragma unroll
+		for (int q = 0; q < 6; q++)
+			wl[q] = Ws[(q * 10 + i) * 32 + c];
+		if (nb < 0 && bflux == GDH_FLUX_MAXWELL_METAL)
+			maxwell_metal(wl, vn, fl);
+		else
+			maxwell_upwind(wl, wra[bp], vn, overr, fl);
+#pragma unroll
+		for (int p = 0; p < 3; p++)
+			swap_halves(fl[p], fl[3 + p]);
+#pragma unroll
+		for (int m = 0; m < 10; m++) {
+			const T la = Tb->minv_lift[F][m][bp], lb = Tb->minv_lift[F][m][3 + bp];
+#pragma unroll
+			for (int p = 0; p < 3; p++)
+				acc[p][m] += la * fl[p] + lb * fl[3 + p];
+		}
+		pin3(acc);
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_rk3_pair(
+	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
+	const T *__restrict__ GH, const double *__restrict__ G,
+	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
+	const Rk3Tables<T> *__restrict__ Tb, int ne_, int ngh_, double Ac,
+	double Bc, double dt, int first, int bflux, const int *__restrict__ inst,
+	const double *__restrict__ dt_tab, const double *__restrict__ ieps)
+{
+	__shared__ T Ws[60 * 32]; /* value (q, i) of tile column c at (10 q + i) 32 + c */
+	const long ne = ne_, ngh = ngh_;
+	const int tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int c = threadIdx.x % 32, h = threadIdx.x / 32, c0 = tile * 32, e = c0 + c;
+	const bool live = e < ne_;
+	{
+		/* each half stages 30 rows, all loads in flight */
+		T v[30];
+#pragma unroll
+		for (int r = 0; r < 30; r++)
+			v[r] = live ? W[(30 * h + r) * ne + e] : T(0);
+#pragma unroll
+		for (int r = 0; r < 30; r++)
+			Ws[(30 * h + r) * 32 + c] = v[r];
+	}
+	__syncthreads();
+	if (!live)
+		return;
+	double ctd[9];
+#pragma unroll
+	for (int k = 0; k < 9; k++)
+		ctd[k] = G[k * ne + e];
+	const double det = G[9 * ne + e];
+	T acc[3][10];
+#pragma unroll
+	for (int p = 0; p < 3; p++)
+#pragma unroll
+		for (int m = 0; m < 10; m++)
+			acc[p][m] = T(0);
+	/* E rows (h = 0) from the H fields, H rows (h = 1) from the E fields */
+	const int qs0 = 3 - 3 * h;
+	const T sigma = h ? T(1) : T(-1);
+	/* each face's neighbour trace is gathered one phase ahead: face 0's
+	 * during the volume term, face f + 1's during face f's flux */
+	const int4 n4 = nbr[e], m4 = nbmap[e];
+	T ga[3][6], gb[3][6];
+	gather_pair<0>(ga, Ws, W, GH, ne, ngh, h, c0, n4.x, m4.x);
+	volume_pair<0>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	volume_pair<1>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	volume_pair<2>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	gather_pair<1>(gb, Ws, W, GH, ne, ngh, h, c0, n4.y, m4.y);
+	face_pair<0>(acc, Ws, ga, c, h, n4.x, ctd, bflux, Tb);
+	gather_pair<2>(ga, Ws, W, GH, ne, ngh, h, c0, n4.z, m4.z);
+	face_pair<1>(acc, Ws, gb, c, h, n4.y, ctd, bflux, Tb);
+	gather_pair<3>(gb, Ws, W, GH, ne, ngh, h, c0, n4.w, m4.w);
+	face_pair<2>(acc, Ws, ga, c, h, n4.z, ctd, bflux, Tb);
+	face_pair<3>(acc, Ws, gb, c, h, n4.w, ctd, bflux, Tb);
+	const T invdet = (T)(1.0 / det);
+	const T sc = h == 0 && ieps ? (T)(ieps[e] / det) : invdet;
+	const T dtT = (T)(inst ? dt_tab[inst[e]] : dt), AcT = (T)Ac, BcT = (T)Bc;
+#pragma unroll
+	for (int p = 0; p < 3; p++)
+#pragma unroll
+		for (int i = 0; i < 10; i++) {
+			const int r = (3 * h + p) * 10 + i;
+			const long g = r * ne + e;
+			const T d = acc[p][i] * sc;
+			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
+			K2[g] = k;
+			Wo[g] = Ws[r * 32 + c] + BcT * k;
+		}
+}
+
 template <typename T, int BS>
 static void rk3_launch_bs(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 			  double dt, int first, const void *W, void *Wo)
@@ -386,6 +608,14 @@ template <typename T>
 static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 		       double dt, int first, const void *W, void *Wo)
 {
+	if (c->rk3_tile == 2) {
+		k_rk3_pair<T><<<nblocks(c->ne, 32), 64, 0, c->stream>>>(
+			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
+			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
+			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
+		HIPCHECK(hipGetLastError());
+		return;
+	}
 	if (c->rk3_tile) {
 		k_rk3_tile<T><<<nblocks(c->ne, 64), 64, 0, c->stream>>>(
 			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,

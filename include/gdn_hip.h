@@ -191,10 +191,11 @@ void gdh_batch_setup(gdh_ctx *ctx, int ninst, const int *elem_first,
 		     const int *bface_first, const double *par, const double *dt,
 		     const double *eps);
 /* Tile sizes by timing candidates on the batch: out[0] elements per block of
- * the RK3 stage (64: the tile's state staged in LDS; 128/256: one element
- * per thread over the global state), out[1] nodes per block of the
- * relaxation (128/256/512). Call before the run starts (it overwrites the
- * second state buffer). */
+ * the RK3 stage (32: the tile's state staged in LDS, two lanes per element;
+ * 64: the same with one lane per element; 128/256: one element per thread
+ * over the global state; GDN_RK3=pair|tile|soa keeps that kernel's
+ * candidates only), out[1] nodes per block of the relaxation (128/256/512).
+ * Call before the run starts (it overwrites the second state buffer). */
 void gdh_batch_autotune(gdh_ctx *ctx, int out[2]);
 /* nsteps time steps of every instance (RK3: ghosts at t + c_s dt and the
  * stage, three times; kinetic: ghosts at t + dt/2, transport, relaxation),

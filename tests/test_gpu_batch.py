@@ -86,13 +86,19 @@ def run_batch(cases, kinetic, cfl, steps, precision, split=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fused", ["0", "1"])
-@pytest.mark.parametrize("kinetic,cases,cfl", [(None, CASES_RK3, 0.5), ("d3q4", CASES_KIN, 5.0)])
-def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, monkeypatch):
-    """Bit for bit, with the relaxation form fixed (separate, or fused into
-    the sweep: GDN_RELAX_FUSED) for the batch and the single instances."""
+@pytest.mark.parametrize("kinetic,cases,cfl,fused,rk3",
+                         [(None, CASES_RK3, 0.5, "0", "pair"), (None, CASES_RK3, 0.5, "0", "tile"),
+                          (None, CASES_RK3, 0.5, "0", "soa"), ("d3q4", CASES_KIN, 5.0, "0", None),
+                          ("d3q4", CASES_KIN, 5.0, "1", None)])
+def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, rk3, monkeypatch):
+    """Bit for bit, with the RK3 stage kernel (GDN_RK3: the three kernels
+    agree to rounding only) and the relaxation form (separate, or fused into
+    the sweep: GDN_RELAX_FUSED) fixed for the batch and the single
+    instances."""
     monkeypatch.setenv("GDN_RELAX_FUSED", fused)
-    print(f"batch {kinetic or 'rk3'} fused {fused}", flush=True)
+    if rk3:
+        monkeypatch.setenv("GDN_RK3", rk3)
+    print(f"batch {kinetic or 'rk3'} fused {fused} rk3 {rk3}", flush=True)
     steps = 7
     out, tiles = run_batch(cases, kinetic, cfl, steps, "fp64")
     print(f"  tiles {tiles}", flush=True)

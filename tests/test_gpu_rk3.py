@@ -107,8 +107,9 @@ def test_rk3_host_callback_ghosts():
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("boundary", ["upwind", "metal"])
 def test_rk3_tiled_stage_equals_global_stage(precision, boundary, monkeypatch):
-    """The LDS-tiled RK3 stage (default) and the stage over the global state
-    (GDN_RK3=soa) compute the same sums; the compiler fuses and orders the
+    """The LDS-tiled RK3 stages with one lane per element (GDN_RK3=tile, the
+    binary32 default) and two (GDN_RK3=pair, the binary64 default) and the
+    stage over the global state (GDN_RK3=soa) compute the same sums; the compiler fuses and orders the
     floating-point operations of the two kernels differently, so the states
     after a whole solve agree to rounding: relative L2 <= 1e-13 (FP64),
     <= 1e-5 (FP32). 12^3 x 6 = 10368 tets spans 162 tiles of 64 elements,
@@ -121,7 +122,7 @@ def test_rk3_tiled_stage_equals_global_stage(precision, boundary, monkeypatch):
         G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
         imposed = "plane_wave"
     out = {}
-    for k in ("soa", "tile"):
+    for k in ("soa", "tile", "pair"):
         monkeypatch.setenv("GDN_RK3", k)
         mesh = G.Mesh(path=path)
         sim = G.Simulation(mesh, G.Model(imposed, boundary=boundary))
@@ -133,6 +134,7 @@ def test_rk3_tiled_stage_equals_global_stage(precision, boundary, monkeypatch):
         out[k] = sim.wn()
         st.free()
         sim.free()
-    d = rel_l2(out["tile"], out["soa"])
-    print(f"tiled vs global RK3 stage, {precision} {boundary}: rel L2 {d:.3e}", flush=True)
-    assert d <= (1e-13 if precision == "fp64" else 1e-5)
+    for k in ("tile", "pair"):
+        d = rel_l2(out[k], out["soa"])
+        print(f"{k} vs global RK3 stage, {precision} {boundary}: rel L2 {d:.3e}", flush=True)
+        assert d <= (1e-13 if precision == "fp64" else 1e-5)
