@@ -40,10 +40,12 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # bytes, WRITE_SIZE all of them); per element of the workloads measured there
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
-SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # relaxation fused into the sweep (1) or separate (0)
-    1: (2077260.8 / 0.5 + 1108102.5) * 1024 / 196608,
-    0: (1345019.2 / 0.5 + 1108102.5) * 1024 / 196608,
+SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep)
+    ("item", 1): (2077260.8 / 0.5 + 1108102.5) * 1024 / 196608,
+    ("item", 0): (1345019.2 / 0.5 + 1108102.5) * 1024 / 196608,
 }
+SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
+                "cluster": "k_sweep_cluster"}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
     ("fp64", "k_rk3_pair"): (1317320.8 / 0.5 + 1474560.0) * 1024 / 1572864,
     ("fp32", "k_rk3_tile"): (837410.6 / 0.5 + 737280.0) * 1024 / 1572864,
@@ -408,11 +410,13 @@ def main():
     st.sync()
     t_setup = time.perf_counter() - t0
     relax_fused = G.relax_fused(st)
+    sweep = G.sweep_info(st)
+    traffic = SWEEP_TRAFFIC_BYTES_PER_ELEM.get((sweep["form"], relax_fused))
     sim_itermax = sim.s.itermax
     default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
                         and args.precision == "fp64")
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
-          f"levels={st.levels} sweep_grid={G.sweep_grid(st)} "
+          f"levels={st.levels} sweep={sweep} "
           f"relax_fused={relax_fused} setup={t_setup:.2f}s",
           file=sys.stderr, flush=True)
 
@@ -469,6 +473,9 @@ def main():
             "l2_error": err,
             "hbm_gb_s_sweep": achieved,
             "relaxation_fused_into_sweep": bool(relax_fused),
+            "sweep_form": sweep["form"],
+            "sweep_clusters": sweep["clusters"],
+            "sweep_resident_workgroups": sweep["grid"],
         },
         "roofline": {
             "bound": "hbm",
@@ -476,11 +483,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (SWEEP_TRAFFIC_BYTES_PER_ELEM[relax_fused] * nelem
-                        if default_workload else None),
-            "traffic_gb_s": (SWEEP_TRAFFIC_BYTES_PER_ELEM[relax_fused] * nelem / sweep_s / 1e9
-                             if default_workload else None),
-            "kernel": "k_sweep_persistent (one launch = the whole sweep of one step)",
+            "traffic": (traffic * nelem if default_workload and traffic else None),
+            "traffic_gb_s": (traffic * nelem / sweep_s / 1e9
+                             if default_workload and traffic else None),
+            "kernel": f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step)",
             "bytes_per_element_step": sweep_bytes,
             "launches_per_step": launches,
             "avg_launch_us": sweep_s / max(launches, 1) * 1e6,

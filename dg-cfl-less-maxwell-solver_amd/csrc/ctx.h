@@ -118,10 +118,22 @@ struct gdh_ctx {
 			m = std::max(m, cap_off_h[v + 1] - cap_off_h[v]);
 		return m;
 	}
-	int sweep_mode; /* 1 persistent (default), 0 one launch per level */
+	int sweep_mode; /* 2 clusters, 1 persistent items, 0 one launch per level */
 	int sweep_poll; /* 1: waiting items poll whole traces */
 	int relax_pending; /* the state awaits the relaxation of its last step */
 	int relax_fused; /* the sweep applies the relaxation as it reads its input */
+	/* cluster sweep: host copies of the element vertices (binary32, 12 per
+	 * element) and neighbours (mesh order), taken at creation; per
+	 * cluster (v, elements) a header {first item, n | nx << 8, external
+	 * faces}, item descriptors in cluster order */
+	std::vector<float> vtx;
+	std::vector<int> nbr_h;
+	int ncl;
+	int4 *chead;
+	struct ItemDesc *cdesc;
+	int cl_grid;
+	long cl_multi; /* items in clusters of more than one element */
+	int cl_levels;
 
 	/* optional event timing of the main kernels: pairs of events recorded
 	 * on the stream around each kernel group, resolved when read */
@@ -228,6 +240,25 @@ __host__ __device__ inline long kin_at(long ne, long neq, int nv, long ipg, int 
 	const long g = ipg - ne10, a = g % 6;
 	return 60L * nv * ne + 6L * v * (neq - ne10) + 6 * (g - a) + 6 * iw + a;
 }
+
+/* Cluster plan of the transport sweep (cluster.hip): clusters of at most
+ * CL_MAX elements of one velocity with at most CL_XMAX inflow faces from
+ * outside, in a topological order of the cluster graph. */
+#define CL_MAX 6
+#define CL_XMAX 8
+struct ClusterPlan {
+	std::vector<int4> head; /* { first item, n | nx << 8, external faces
+				 * (byte x = 4 j + f: face f of item j) } */
+	std::vector<int> perm; /* cluster-order item -> level-order item */
+	std::vector<int> code; /* per item: 4 bits per face, external face x
+				* (0..7) or 8 + in-cluster producer j, at bit
+				* 12 + 4 f of the descriptor flags */
+	long multi; /* items in clusters of more than one element */
+	int levels; /* longest path of the cluster graphs */
+};
+struct gdh_ctx;
+bool cluster_plan(const gdh_ctx *c, const uint8_t *cls, const int *items_ve,
+		  const int *level_ptr, int nlevels, ClusterPlan &P);
 
 /* launchers implemented in rk3.hip and sweep.hip */
 void sweep_setup(gdh_ctx *c);

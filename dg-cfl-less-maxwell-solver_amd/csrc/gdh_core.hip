@@ -251,6 +251,20 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->nitems = 0;
 	c->relax_pending = 0;
 	c->relax_fused = 0;
+	c->ncl = 0;
+	c->chead = nullptr;
+	c->cdesc = nullptr;
+	c->cl_grid = 0;
+	c->cl_multi = 0;
+	c->cl_levels = 0;
+	if (scheme) {
+		/* vertices and neighbours for the cluster plan of the sweep */
+		c->vtx.resize(12 * ne);
+		for (size_t e = 0; e < ne; e++)
+			for (int k = 0; k < 12; k++)
+				c->vtx[12 * e + k] = (float)mesh->xnode[30 * e + k];
+		c->nbr_h.assign(mesh->nbr, mesh->nbr + 4 * ne);
+	}
 	return c;
 }
 
@@ -264,7 +278,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->hostvals, c->partials, c->items, c->dinv,
 			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa, c->elem_pos,
 			 c->wx, c->wz, c->cap_ipg, c->cap_vel, c->cap_off, c->cap_piv,
-			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev,
+			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev, c->chead, c->cdesc,
 			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
 			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
 			 c->stage };

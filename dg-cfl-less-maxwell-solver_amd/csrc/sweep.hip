@@ -16,6 +16,8 @@ using namespace gdh;
 
 static void cap_setup(gdh_ctx *c);
 static void relax_choose(gdh_ctx *c);
+static void cluster_setup(gdh_ctx *c, const uint8_t *cls, const int *items_ve,
+			  const int *level_ptr, int nlevels);
 
 /* own outward flux coefficient s_f = v . vn(e, f) */
 __device__ inline double face_coef(const double ct[9], int f, const double *v)
@@ -152,11 +154,20 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 		c->nlagged += cls[k] == 3;
 	c->lag_last = c->lag_max = 0;
 	sweep_setup(c);
+	const int mode = c->sweep_mode;
+	c->sweep_mode = mode == 0 ? 0 : 1; /* the capacitance set-up sweeps by items */
 	if (c->nlagged) {
 		const size_t blen = (size_t)c->m * c->neq;
 		c->wx = dmalloc(c->rsz * blen);
 		cap_setup(c);
 	}
+	if (mode != 0 && mode != 1)
+		cluster_setup(c, cls, items_ve, level_ptr, nlevels);
+	if (mode == 2 && !c->ncl) {
+		fprintf(stderr, "gdon3d-hip: GDN_SWEEP=cluster, but the mesh gives no cluster plan\n");
+		abort();
+	}
+	c->sweep_mode = mode;
 	relax_choose(c);
 }
 
@@ -180,8 +191,11 @@ struct ItemDesc {
 	double cf[4]; /* s_f dt / det on the inflow faces, 0 elsewhere */
 	int e, v;
 	int flags; /* bit f: inflow face f; bit 4 + f: inflow from the ghosts;
-		    * bit 8 + f: lagged inflow (cycle broken; value given in xl) */
-	int pad;
+		    * bit 8 + f: lagged inflow (cycle broken; value given in xl);
+		    * cluster order only, bits 12 + 4 f .. 15 + 4 f: where the
+		    * trace of face f comes from, inflow face x (0..7) of the
+		    * cluster from outside, or 8 + the in-cluster producer */
+	int pad; /* cluster order only: the item's level-order index */
 	int nb[4]; /* neighbour element, or the boundary face index */
 	int map[4]; /* 4 bits per face node: the neighbour's local node */
 };
@@ -301,6 +315,7 @@ struct ItemBuf {
 			     * ten rows read at once fall in distinct banks) */
 	int v, e, nslot, pad;
 	int nb[3], map[3];
+	int loc[3]; /* cluster sweep: source code of the slot's trace */
 };
 
 /* Per-block scratch of one item's preparation: D^-1, and 60 values (one
@@ -432,6 +447,7 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 				continue;
 			B.nb[s] = d.nb[f];
 			B.map[s] = d.map[f];
+			B.loc[s] = (d.flags >> (12 + 4 * f)) & 15;
 			s++;
 		}
 		B.nslot = s;
@@ -619,11 +635,11 @@ __device__ inline bool spin_expired(int spins, int *status)
 /* sol = base - sum_s G_s X_s with X[s] the published trace w of slot s's
  * neighbour; sol goes to fn and w = (1 - theta) fv + theta sol (fv, base:
  * the lane's old and base values) to the hand-off words, each rounded once
- * to R. */
+ * to R, and (cluster sweep) the rounded w to wl[t]. */
 template <typename R>
 __device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], R fv,
 				    double base, R *__restrict__ fn, u64 *ll, int ne,
-				    unsigned epoch)
+				    unsigned epoch, double *wl = nullptr)
 {
 	const int t = lane_id();
 	const int iw = t / 10, i = t % 10;
@@ -636,8 +652,10 @@ __device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], 
 				sol -= B.G[s][i][b] * X[s][6 * iw + b];
 		const int v = uni(B.v), e = uni(B.e);
 		fn[((long)v * ne + e) * 60 + t] = (R)sol;
-		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t,
-				(R)((1.0 - THETA) * (double)fv + THETA * sol), epoch);
+		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
+		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
+		if (wl)
+			wl[t] = (double)wr;
 	}
 	__syncthreads();
 }
@@ -804,6 +822,365 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 	}
 }
 
+/* ------------------------------------------------------------------ */
+/* Cluster sweep                                                          */
+/* ------------------------------------------------------------------ */
+
+/* dword w (uniform, below 128) of a cluster's descriptors held one dword
+ * per lane: lane l holds dwords l (a) and 64 + l (b) */
+__device__ inline int cl_dword(int a, int b, int w)
+{
+	const int x = __builtin_amdgcn_readlane(a, w & 63);
+	const int y = __builtin_amdgcn_readlane(b, w & 63);
+	return w < 64 ? x : y;
+}
+
+/* descriptor of item j (a compile-time index once unrolled) */
+__device__ inline ItemDesc cl_desc(int a, int b, int j)
+{
+	int w[20];
+#pragma unroll
+	for (int k = 0; k < 20; k++) {
+		const int x = 20 * j + k;
+		w[k] = x < 64 ? __builtin_amdgcn_readlane(a, x) : __builtin_amdgcn_readlane(b, x - 64);
+	}
+	ItemDesc d;
+	__builtin_memcpy(&d, w, sizeof d);
+	return d;
+}
+
+/* the descriptors of the n items from item i0 on, one dword per lane */
+__device__ inline void cl_load_desc(const ItemDesc *__restrict__ cdesc, int i0, int n, int &a,
+				    int &b)
+{
+	const int t = lane_id();
+	const int *p = reinterpret_cast<const int *>(cdesc + i0);
+	a = t < 20 * n ? p[t] : 0;
+	b = 64 + t < 20 * n ? p[64 + t] : 0;
+}
+
+/*
+ * One pass over the inflow faces x of the cluster in um (uniform) whose
+ * trace lane t = 6 q + b (t < 36) still misses (bit x of pend): a complete
+ * value goes to Xc[x][t]. Face x is face f of item j, byte x of the
+ * header's words z, w holding 4 j + f. All loads are issued before any is
+ * decoded. Returns the faces still pending in this lane.
+ */
+template <typename R>
+__device__ inline unsigned cl_poll(int hz, int hw, int da, int db, unsigned um, unsigned pend,
+				   u64 *ll, int v, int ne, unsigned epoch, double (*Xc)[36])
+{
+	const int t = lane_id();
+	const int q = t / 6, b = t % 6;
+	u64 w[CL_XMAX][2];
+#pragma unroll
+	for (int x = 0; x < CL_XMAX; x++) {
+		w[x][0] = w[x][1] = 0;
+		if (!(um & (1u << x)))
+			continue;
+		const int jf = uni(((x < 4 ? hz : hw) >> (8 * (x & 3))) & 255);
+		const int j = jf >> 2, f = jf & 3;
+		const int nb = cl_dword(da, db, 20 * j + 12 + f);
+		const int map = cl_dword(da, db, 20 * j + 16 + f);
+		if (t < 36 && (pend & (1u << x)))
+			Handoff<R>::load(ll_item<R>(ll, v, nb, ne), q * 10 + ((map >> (4 * b)) & 15),
+					 w[x]);
+	}
+#pragma unroll
+	for (int x = 0; x < CL_XMAX; x++) {
+		double val;
+		if ((um & (1u << x)) && t < 36 && (pend & (1u << x)) &&
+		    Handoff<R>::decode(w[x], epoch, val)) {
+			Xc[x][t] = val;
+			pend &= ~(1u << x);
+		}
+	}
+	return pend;
+}
+
+/* faces with a bit of pend set in some lane (uniform) */
+__device__ inline unsigned cl_any(unsigned pend)
+{
+	unsigned um = 0;
+#pragma unroll
+	for (int x = 0; x < CL_XMAX; x++)
+		um |= __ballot((pend >> x) & 1u) ? 1u << x : 0u;
+	return um;
+}
+
+/* every block records a 100 us busy interval (residency measurement) */
+__device__ inline void residency_probe(u64 *disc)
+{
+	if (threadIdx.x == 0) {
+		const u64 t0 = __builtin_amdgcn_s_memrealtime(); /* 100 MHz */
+		u64 t1 = t0;
+		for (int n = 0; n < (1 << 26) && t1 - t0 < 10000; n++)
+			t1 = __builtin_amdgcn_s_memrealtime();
+		disc[2 * blockIdx.x] = t0;
+		disc[2 * blockIdx.x + 1] = t1;
+	}
+}
+
+/*
+ * Transport of one item of a cluster, in direct form once its inflow
+ * traces are known (the item sweep instead forms D^-1 ML_s per slot before
+ * its wait):
+ *   sol = D^-1 (2 f - sum_f c_f ML_f u_f) - f
+ * over every inflow face f in face order, u_f the ghost value, the lagged
+ * value (1 - theta) f_nb + theta x_nb, the published trace of a producer
+ * outside the cluster (Xc[x]) or of an earlier item of the cluster (Wc[j]).
+ * Writes sol to fn, w = (1 - theta) f + theta sol to the hand-off words and,
+ * rounded alike, to wl. Lanes t = 10 iw + i (t < 60).
+ */
+template <typename R>
+__device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
+			       const R *__restrict__ fo, const R *__restrict__ xl,
+			       R *__restrict__ fn, u64 *ll, int ne, int neq, int nv, unsigned epoch,
+			       const SweepTables &T, PrepScratch &W, const double (*Wc)[60],
+			       const double (*Xc)[36], double *wl)
+{
+	const int t = lane_id();
+	const int iw = t / 10, i = t % 10;
+	const R fv = P.fr[0];
+	W.D[t / 10][t % 10] = P.d0;
+	if (t < 36)
+		W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
+	double y = 0.0;
+#pragma unroll
+	for (int f = 0; f < 4; f++) {
+		if (!(d.flags & (1 << f)))
+			continue;
+		const double cf = d.cf[f];
+		double s = 0.0;
+		if (d.flags & (0x110 << f)) {
+			/* ghost or lagged: the trace staged through W.S */
+			if (t < 36) {
+				const long g = trace_index(d, f, t, ne, neq, nv);
+				double u = (double)fo[g];
+				if (!(d.flags & (16 << f))) {
+					u = (1.0 - THETA) * u;
+					if (xl)
+						u += THETA * (double)xl[g];
+				}
+				W.S[t] = u;
+			}
+			__syncthreads();
+			if (t < 60)
+#pragma unroll
+				for (int b = 0; b < 6; b++)
+					s += T.ML[f][i][b] * W.S[6 * iw + b];
+			__syncthreads();
+		} else {
+			const int code = (d.flags >> (12 + 4 * f)) & 15;
+			if (t < 60) {
+				if (code >= 8) {
+					const int m = d.map[f];
+#pragma unroll
+					for (int b = 0; b < 6; b++)
+						s += T.ML[f][i][b] *
+						     Wc[code - 8][10 * iw + ((m >> (4 * b)) & 15)];
+				} else {
+#pragma unroll
+					for (int b = 0; b < 6; b++)
+						s += T.ML[f][i][b] * Xc[code][6 * iw + b];
+				}
+			}
+		}
+		y += cf * s;
+	}
+	if (t < 60)
+		W.S[t] = 2.0 * (double)fv - y;
+	__syncthreads();
+	if (t < 60) {
+		double s = 0.0;
+#pragma unroll
+		for (int l = 0; l < 10; l++)
+			s += W.D[i][l] * W.S[10 * iw + l];
+		const double sol = s - (double)fv;
+		fn[((long)d.v * ne + d.e) * 60 + t] = (R)sol;
+		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
+		Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
+		wl[t] = (double)wr;
+	}
+	__syncthreads();
+}
+
+/*
+ * The sweep over clusters in one launch. Cluster k (header chead[k]: first
+ * item, n | nx << 8, inflow faces from outside) holds n <= CL_MAX items of
+ * one velocity in level order; block b takes clusters b, b + G, ... in a
+ * topological order of the cluster graphs, so that the earliest unfinished
+ * cluster always has its inputs from outside ready (all G blocks are
+ * co-resident; bounded waits report a timeout in *status). At the start of
+ * a cluster the loads of all its items and a first poll of its nx inflow
+ * faces from outside are issued together (one round trip); the items are
+ * then solved in order (cl_item), taking the traces of in-cluster producers
+ * from LDS (Wc) and the others from Xc once complete. Descriptors are read
+ * one cluster ahead, headers two.
+ */
+template <typename R>
+__global__ __launch_bounds__(64, 4) void k_sweep_cluster(
+	const int4 *__restrict__ chead, const ItemDesc *__restrict__ cdesc, int ncl,
+	const R *__restrict__ fo, R *__restrict__ fn, const R *__restrict__ xl,
+	const double *__restrict__ dinv, int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
+	const unsigned *__restrict__ epoch_dev, int *status, u64 *disc, KinPar kp)
+{
+	__shared__ SweepTables T;
+	__shared__ PrepScratch W;
+	__shared__ double Wc[CL_MAX][60];
+	__shared__ double Xc[CL_XMAX][36];
+	if (disc) {
+		residency_probe(disc);
+		return;
+	}
+	load_tables(T);
+	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
+	const int G = gridDim.x;
+	int ci = blockIdx.x;
+	if (ci >= ncl)
+		return;
+	const int t = threadIdx.x;
+	int4 h = chead[ci];
+	int da, db;
+	cl_load_desc(cdesc, h.x, h.y & 255, da, db);
+	int4 hn = make_int4(0, 0, 0, 0);
+	if (ci + G < ncl)
+		hn = chead[ci + G];
+	for (; ci < ncl; ci += G) {
+		const int n = uni(h.y & 255), nx = uni((h.y >> 8) & 255);
+		const int hz = uni(h.z), hw = uni(h.w);
+		/* descriptors of the next cluster, header of the one after */
+		int nda = 0, ndb = 0;
+		if (ci + G < ncl)
+			cl_load_desc(cdesc, uni(hn.x), uni(hn.y & 255), nda, ndb);
+		int4 hnn = make_int4(0, 0, 0, 0);
+		if (ci + 2 * G < ncl)
+			hnn = chead[ci + 2 * G];
+		PrepLoads<R, 1> P[CL_MAX];
+#pragma unroll
+		for (int j = 0; j < CL_MAX; j++)
+			if (j < n) {
+				const ItemDesc d = cl_desc(da, db, j);
+				prep_issue<R, 1>(d.pad, d, fo, dinv, ne, 0, P[j]);
+			}
+		const int v = cl_dword(da, db, 9);
+		const unsigned all = (1u << nx) - 1u;
+		unsigned pend = cl_poll<R>(hz, hw, da, db, all, t < 36 ? all : 0u, ll, v, ne,
+					   epoch, Xc);
+#pragma unroll
+		for (int j = 0; j < CL_MAX; j++) {
+			if (j >= n)
+				break;
+			const ItemDesc d = cl_desc(da, db, j);
+			/* inflow faces from outside that this item reads */
+			unsigned need = 0;
+			const int sf = slot_faces(d);
+#pragma unroll
+			for (int f = 0; f < 4; f++) {
+				const int code = (d.flags >> (12 + 4 * f)) & 15;
+				if ((sf & (1 << f)) && code < 8)
+					need |= 1u << code;
+			}
+			int spins = 0;
+			for (unsigned um = cl_any(pend & need); um; um = cl_any(pend & need)) {
+				__builtin_amdgcn_s_sleep(1);
+				pend = cl_poll<R>(hz, hw, da, db, um, pend, ll, v, ne, epoch, Xc);
+				if (spin_expired(++spins, status))
+					break;
+			}
+			cl_item<R>(d, P[j], fo, xl, fn, ll, ne, neq, nv, epoch, T, W, Wc, Xc, &Wc[j][0]);
+		}
+		h = hn;
+		hn = hnn;
+		da = nda;
+		db = ndb;
+	}
+}
+
+/* largest number of the g0 intervals [h[2b], h[2b + 1]) that overlap */
+static int max_overlap(const std::vector<u64> &h, int g0)
+{
+	std::vector<std::pair<u64, int>> ev;
+	ev.reserve(2 * (size_t)g0);
+	for (int b = 0; b < g0; b++) {
+		ev.push_back({ h[2 * b], 1 });
+		ev.push_back({ h[2 * b + 1], -1 });
+	}
+	std::sort(ev.begin(), ev.end(), [](const std::pair<u64, int> &x,
+					   const std::pair<u64, int> &y) {
+		return x.first < y.first || (x.first == y.first && x.second < y.second);
+	});
+	int cur = 0, best = 0;
+	for (const auto &e : ev) {
+		cur += e.second;
+		best = std::max(best, cur);
+	}
+	return std::max(1, std::min(best, g0));
+}
+
+template <typename R> static int cluster_residency(gdh_ctx *c)
+{
+	int dev = 0, ncu = 0, per_cu = 0;
+	HIPCHECK(hipGetDevice(&dev));
+	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_cluster<R>, 64, 0));
+	const int g0 = std::max(1, per_cu * ncu);
+	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
+	k_sweep_cluster<R><<<g0, 64, 0, c->stream>>>(nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+						     nullptr, 0, 0, 0, nullptr, 0, nullptr, nullptr,
+						     disc, c->kp);
+	HIPCHECK(hipGetLastError());
+	std::vector<u64> h(2 * (size_t)g0);
+	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(), hipMemcpyDeviceToHost,
+				c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	HIPCHECK(hipFree(disc));
+	return max_overlap(h, g0);
+}
+
+/* cluster-order descriptors: the level-order descriptor of each item, its
+ * level-order index in pad and the trace sources in the flags */
+__global__ void k_cluster_desc(const ItemDesc *__restrict__ desc, const int *__restrict__ perm,
+			       const int *__restrict__ code, int n, ItemDesc *__restrict__ cdesc)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	ItemDesc d = desc[perm[k]];
+	d.pad = perm[k];
+	d.flags |= code[k] << 12;
+	cdesc[k] = d;
+}
+
+/* the cluster plan on the device, and the blocks the cluster sweep holds
+ * at once; no plan (ncl = 0) when the mesh gives none */
+static void cluster_setup(gdh_ctx *c, const uint8_t *cls, const int *items_ve,
+			  const int *level_ptr, int nlevels)
+{
+	c->ncl = 0;
+	ClusterPlan P;
+	if (!cluster_plan(c, cls, items_ve, level_ptr, nlevels, P) || P.head.empty())
+		return;
+	c->ncl = (int)P.head.size();
+	c->cl_multi = P.multi;
+	c->cl_levels = P.levels;
+	c->chead = (int4 *)dmalloc(sizeof(int4) * P.head.size());
+	HIPCHECK(hipMemcpy(c->chead, P.head.data(), sizeof(int4) * P.head.size(),
+			   hipMemcpyHostToDevice));
+	int *perm = (int *)dmalloc(sizeof(int) * P.perm.size());
+	int *code = (int *)dmalloc(sizeof(int) * P.code.size());
+	HIPCHECK(hipMemcpy(perm, P.perm.data(), sizeof(int) * P.perm.size(), hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(code, P.code.data(), sizeof(int) * P.code.size(), hipMemcpyHostToDevice));
+	c->cdesc = (ItemDesc *)dmalloc(sizeof(ItemDesc) * (size_t)c->nitems);
+	k_cluster_desc<<<nblocks(c->nitems, 256), 256, 0, c->stream>>>(c->desc, perm, code,
+								       c->nitems, c->cdesc);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	HIPCHECK(hipFree(perm));
+	HIPCHECK(hipFree(code));
+	c->cl_grid = c->fp32 ? cluster_residency<float>(c) : cluster_residency<double>(c);
+}
+
 /*
  * Blocks of k_sweep_persistent that the device holds at once, measured: the
  * occupancy query over-counts on gfx950 when LDS bounds the residency (it
@@ -844,23 +1221,7 @@ template <typename R> static int sweep_residency(gdh_ctx *c)
 				hipMemcpyDeviceToHost, c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	HIPCHECK(hipFree(disc));
-	/* largest overlap of the intervals [t0, t1) */
-	std::vector<std::pair<u64, int>> ev;
-	ev.reserve(2 * (size_t)g0);
-	for (int b = 0; b < g0; b++) {
-		ev.push_back({ h[2 * b], 1 });
-		ev.push_back({ h[2 * b + 1], -1 });
-	}
-	std::sort(ev.begin(), ev.end(), [](const std::pair<u64, int> &x,
-					   const std::pair<u64, int> &y) {
-		return x.first < y.first || (x.first == y.first && x.second < y.second);
-	});
-	int cur = 0, best = 0;
-	for (const auto &e : ev) {
-		cur += e.second;
-		best = std::max(best, cur);
-	}
-	return std::max(1, std::min(best, g0));
+	return max_overlap(h, g0);
 }
 
 void sweep_setup(gdh_ctx *c)
@@ -877,8 +1238,12 @@ void sweep_setup(gdh_ctx *c)
 	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	c->epoch = 0;
+	/* GDN_SWEEP=level|item (or persistent)|cluster fixes the form of the sweep; by default
+	 * the item and cluster sweeps are timed at set-up (relax_choose) */
 	const char *mode = getenv("GDN_SWEEP");
-	c->sweep_mode = (mode && !strcmp(mode, "level")) ? 0 : 1;
+	c->sweep_mode = !mode ? -1 : !strcmp(mode, "level") ? 0
+			: (!strcmp(mode, "item") || !strcmp(mode, "persistent")) ? 1
+			: !strcmp(mode, "cluster") ? 2 : -1;
 	/* waiting items poll one word per inflow face, then the traces
 	 * ("sentinel", the binary64 default), or the whole traces on every
 	 * pass ("full", the binary32 default: one word per value); measured
@@ -936,6 +1301,28 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 	}
 	c->prof_launches++;
 	const R *fo_c = fo;
+	if (c->sweep_mode == 2) {
+		if (relax_in) {
+			fprintf(stderr, "gdon3d-hip: the cluster sweep takes a relaxed state\n");
+			abort();
+		}
+		const int4 *chead = c->chead;
+		const ItemDesc *cdesc = c->cdesc;
+		int ncl = c->ncl;
+		const int cgrid = std::min(c->cl_grid, ncl);
+		if (edev) {
+			k_sweep_cluster<R><<<cgrid, 64, 0, c->stream>>>(
+				chead, cdesc, ncl, fo_c, fn, xl, dinv, nv, ne, neq, ll, 0u, edev,
+				status, disc, kp);
+			HIPCHECK(hipGetLastError());
+			return;
+		}
+		void *cargs[] = { &chead, &cdesc, &ncl, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq,
+				  &ll, &epoch, &edev, &status, &disc, &kp };
+		HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_cluster<R>, dim3(cgrid),
+						    dim3(64), cargs, 0, c->stream));
+		return;
+	}
 	const int grid = std::min(c->sweep_grid, nitems);
 	const SweepKernel<R> kern = sweep_kernel<R>(c);
 	if (edev) {
@@ -1375,43 +1762,74 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 }
 
 /*
- * Fused or separate relaxation, by measurement on this mesh: the fused
- * form saves the relaxation kernel's pass over the state but gives every
- * sweep item the loads of all velocities at its nodes and the relaxation
- * arithmetic. Where the sweep is bound by its dependency chain that work
- * is hidden, where it is bound by its item rate it is not (measured on
- * MI355X, D3Q4 FP64: 1.00 vs 1.03 ms per step at 32^3, 7.06 vs 6.57 ms at
- * 64^3). Both forms are timed (five sweeps each, after one) from the
- * current state into the other buffer; nothing of the state changes.
+ * Form of the sweep and of the relaxation, by measurement on this mesh.
+ * The item sweep (one item per workgroup, every value handed over through
+ * memory) can apply the relaxation as it reads its input: that saves the
+ * relaxation kernel's pass over the state but gives every sweep item the
+ * loads of all velocities at its nodes and the relaxation arithmetic, which
+ * is hidden where the sweep is bound by its dependency chain and not where
+ * it is bound by its item rate (measured on MI355X, D3Q4 FP64: 1.00 vs 1.03
+ * ms per step at 32^3, 7.06 vs 6.57 ms at 64^3). The cluster sweep takes a
+ * relaxed state. Each candidate (item sweep with the relaxation separate or
+ * fused, cluster sweep with it separate) is timed, five steps after one,
+ * from the current state into the other buffer; nothing of the state
+ * changes. GDN_SWEEP and GDN_RELAX_FUSED=0|1 restrict the candidates.
  */
 static void relax_choose(gdh_ctx *c)
 {
-	c->relax_fused = 0;
 	const char *env = getenv("GDN_RELAX_FUSED");
-	if (c->nlagged || (env && !strcmp(env, "0")))
-		return;
-	if (env && !strcmp(env, "1")) {
-		c->relax_fused = 1;
-		return;
-	}
-	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
-	const int prof = c->prof;
-	c->prof = 0;
-	float ms[2];
-	for (int fused = 0; fused < 2; fused++) {
-		for (int r = 0; r < 6; r++) {
-			if (r == 1)
-				HIPCHECK(hipEventRecord(c->ev0, c->stream));
-			sweep_step(c, fo, fn, nullptr, fused);
-			if (!fused)
-				relax_launch(c, fn, 10L * c->ne);
+	const int fz_only = (c->nlagged || (env && !strcmp(env, "0"))) ? 0
+			    : (env && !strcmp(env, "1"))		    ? 1
+									    : -1;
+	int cand[3][2], nc = 0; /* (sweep mode, fused) */
+	for (int mode = 0; mode <= 2; mode++) {
+		if (c->sweep_mode >= 0 ? mode != c->sweep_mode : mode == 0)
+			continue;
+		if (mode == 2 && !c->ncl)
+			continue;
+		for (int fused = 0; fused < 2; fused++) {
+			if ((fz_only >= 0 && fused != fz_only) || (mode == 2 && fused))
+				continue;
+			cand[nc][0] = mode;
+			cand[nc][1] = fused;
+			nc++;
 		}
-		HIPCHECK(hipEventRecord(c->ev1, c->stream));
-		HIPCHECK(hipEventSynchronize(c->ev1));
-		HIPCHECK(hipEventElapsedTime(&ms[fused], c->ev0, c->ev1));
 	}
-	c->prof = prof;
-	c->relax_fused = ms[1] < ms[0];
+	if (nc == 0) {
+		fprintf(stderr, "gdon3d-hip: no form of the sweep fits GDN_SWEEP / GDN_RELAX_FUSED\n");
+		abort();
+	}
+	int best = 0;
+	if (nc > 1) {
+		void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
+		const int prof = c->prof;
+		c->prof = 0;
+		float bms = 1e30f;
+		for (int k = 0; k < nc; k++) {
+			c->sweep_mode = cand[k][0];
+			for (int r = 0; r < 6; r++) {
+				if (r == 1)
+					HIPCHECK(hipEventRecord(c->ev0, c->stream));
+				sweep_step(c, fo, fn, nullptr, cand[k][1]);
+				if (!cand[k][1])
+					relax_launch(c, fn, 10L * c->ne);
+			}
+			HIPCHECK(hipEventRecord(c->ev1, c->stream));
+			HIPCHECK(hipEventSynchronize(c->ev1));
+			float ms = 0;
+			HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+			if (getenv("GDN_VERBOSE"))
+				fprintf(stderr, "gdon3d-hip: sweep form %d, relaxation fused %d: %.3f ms per step\n",
+					cand[k][0], cand[k][1], ms / 5);
+			if (ms < bms) {
+				bms = ms;
+				best = k;
+			}
+		}
+		c->prof = prof;
+	}
+	c->sweep_mode = cand[best][0];
+	c->relax_fused = cand[best][1];
 }
 
 extern "C" int gdh_relax_fused(gdh_ctx *c)
@@ -1440,7 +1858,16 @@ extern "C" void gdh_apply_relaxation(gdh_ctx *c)
 
 extern "C" int gdh_sweep_grid(gdh_ctx *c)
 {
-	return c->sweep_grid;
+	return c->sweep_mode == 2 ? c->cl_grid : c->sweep_grid;
+}
+
+extern "C" void gdh_sweep_info(gdh_ctx *c, int out[5])
+{
+	out[0] = c->sweep_mode;
+	out[1] = c->ncl;
+	out[2] = c->cl_levels;
+	out[3] = (int)std::min(c->cl_multi, (long)2147483647);
+	out[4] = c->sweep_mode == 2 ? c->cl_grid : c->sweep_grid;
 }
 
 extern "C" void gdh_transport_stats(gdh_ctx *c, int out[3])

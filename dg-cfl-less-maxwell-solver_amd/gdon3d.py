@@ -175,6 +175,7 @@ def load_library(path=LIB_PATH):
         "gdh_learned_data": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_float_p, c_float_p]),
         "gdh_sweep_grid": (C.c_int, [C.c_void_p]),
         "gdh_relax_fused": (C.c_int, [C.c_void_p]),
+        "gdh_sweep_info": (None, [C.c_void_p, C.POINTER(C.c_int)]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
@@ -472,6 +473,21 @@ def sweep_grid(stepper):
         raise RuntimeError("stepper already freed")
     lib = load_library()
     return lib.gdh_sweep_grid(lib.dg_stepper_device_context(stepper.h))
+
+
+SWEEP_FORMS = {0: "level", 1: "item", 2: "cluster"}
+
+
+def sweep_info(stepper):
+    """Form of the transport sweep: dict(form, clusters, cluster_levels,
+    clustered_items, grid) (see gdh_sweep_info)."""
+    if not stepper.h:
+        raise RuntimeError("stepper already freed")
+    lib = load_library()
+    out = (C.c_int * 5)()
+    lib.gdh_sweep_info(lib.dg_stepper_device_context(stepper.h), out)
+    return dict(form=SWEEP_FORMS.get(out[0], str(out[0])), clusters=out[1],
+                cluster_levels=out[2], clustered_items=out[3], grid=out[4])
 
 
 def relax_fused(stepper):

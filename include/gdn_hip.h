@@ -177,6 +177,31 @@ int gdh_sweep_grid(gdh_ctx *ctx);
  * forces one); the results agree to rounding. No counterpart in the
  * reference. */
 int gdh_relax_fused(gdh_ctx *ctx);
+/* Form of the transport sweep (no counterpart in the reference; the
+ * reference solves with KLU, src/maths/sparse.c:411-466): out[0] = 0 one
+ * launch per upwind level, 1 one item per workgroup, 2 clusters (the
+ * elements of a grid cell of one velocity solved by one wavefront in level
+ * order, values passed inside the cluster through LDS); out[1] = clusters,
+ * out[2] = levels of the cluster graphs, out[3] = items in clusters of more
+ * than one element, out[4] = resident workgroups of the chosen sweep.
+ * Chosen at gdh_relax_setup by timing (GDN_SWEEP=level|item|cluster forces
+ * one); all forms give bit-identical results. */
+void gdh_sweep_info(gdh_ctx *ctx, int out[5]);
+
+/* The cluster plan of the sweep on the host, without a device (no
+ * counterpart in the reference): ne elements, nv velocities, vtx the 4
+ * vertices of each element (12 floats), nbr the neighbours (nelem * 4, as
+ * gdh_mesh_desc.nbr), ranges[0..nranges] the element ranges of batch
+ * instances (nranges = 0: one range), cls / items_ve / level_ptr / nlevels
+ * the schedule of gdn_sweep_schedule. Fills head (4 ints per cluster: first
+ * item, n | nx << 8, inflow faces from outside as bytes 4 j + f), perm
+ * (cluster-order item -> level-order item, nv * ne) and code (per item, 4
+ * bits per face: external face x < 8 or 8 + in-cluster producer);
+ * stats = { levels of the cluster graphs, items in clusters of more than one
+ * element }. Returns the number of clusters (at most nv * ne), or -1. */
+int gdh_cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const int *ranges,
+		     int nranges, const uint8_t *cls, const int *items_ve, const int *level_ptr,
+		     int nlevels, int *head, int *perm, int *code, int stats[2]);
 
 /* ---- batches (no counterpart in the reference: one drop-in simulation
  * per call there). The context's mesh is the disjoint union of ninst

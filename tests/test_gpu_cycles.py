@@ -77,3 +77,19 @@ def test_cycles_persistent_equals_level(monkeypatch):
         out[mode] = gpu(path, "d3q4", 0.2, 5.0)
     assert out["level"]["stats"] == out["persistent"]["stats"]
     assert np.array_equal(out["level"]["wn"], out["persistent"]["wn"])
+
+
+@pytest.mark.gpu
+def test_cycles_cluster_equals_level(monkeypatch):
+    """Over upwind cycles (two sweeps per step, lagged traces given) the
+    cluster sweep computes what the level launches compute, to rounding
+    (its block solve is written directly; relative L2 <= 1e-13); the
+    clusters that lie on a cycle of the cluster graph are solved as single
+    elements."""
+    path = twisted_msh(4, 4.0)
+    out = {}
+    for mode in ("level", "cluster"):
+        monkeypatch.setenv("GDN_SWEEP", mode)
+        out[mode] = gpu(path, "d3q4", 0.2, 5.0)
+    assert out["level"]["stats"] == out["cluster"]["stats"]
+    assert rel_l2(out["cluster"]["wn"], out["level"]["wn"]) <= 1e-13

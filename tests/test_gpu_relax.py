@@ -125,3 +125,61 @@ def test_fused_relaxation_equals_separate(kin, precision, monkeypatch):
           f"end {rel_l2(b[1], a[1]):.3e}", flush=True)
     assert rel_l2(b[0], a[0]) <= tol and rel_l2(b[1], a[1]) <= tol
     assert abs(b[2] - a[2]) <= tol * a[2] and abs(b[3] - a[3]) <= tol * a[3]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,jitter,tmax,kin,precision", [
+    (4, 0.0, 0.6, "d3q4", "fp64"), (6, 0.2, 0.6, "d3q4", "fp64"), (16, 0.1, 0.1, "d3q4", "fp64"),
+    (8, 0.3, 0.3, "d3q4", "fp64"), (5, 0.1, 0.4, "d3q6", "fp64"), (6, 0.2, 0.4, "d3q4", "fp32"),
+    (16, 0.1, 0.1, "d3q4", "fp32")])
+def test_cluster_sweep_equals_level_sweep(n, jitter, tmax, kin, precision, monkeypatch):
+    """The cluster sweep (the elements of one grid cell of one velocity
+    solved in level order by one wavefront, values inside the cluster passed
+    through LDS, values between clusters through the epoch-tagged hand-off)
+    computes what the level-by-level launches compute, in both precisions
+    and both velocity sets. Its block solve is written directly,
+    D^-1 (2 f - sum_f c_f ML_f u_f) - f, where the level and item sweeps
+    form D^-1 ML_s before waiting, so the two agree to rounding: relative L2
+    <= 1e-13 (FP64), <= 1e-5 (FP32, state rounded to binary32 every step).
+    At 16^3 every resident workgroup takes several clusters, so the
+    look-ahead of descriptors and headers across clusters is exercised; at
+    +-30 % jitter some clusters are split."""
+    path = kuhn_msh(n, jitter, 31)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    out = {}
+    for mode in ("level", "cluster"):
+        monkeypatch.setenv("GDN_SWEEP", mode)
+        mesh = G.Mesh(path=path)
+        sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic=kin))
+        sim.set_time(tmax, 0.0, 5.0)
+        st = G.Stepper(sim, precision=precision)
+        info = G.sweep_info(st)
+        assert info["form"] == mode
+        st.advance(sim.s.itermax)
+        st.sync()
+        st.finish()
+        out[mode] = (sim.wn(), info)
+        st.free()
+        sim.free()
+    info = out["cluster"][1]
+    print(f"cluster sweep {n}^3 +-{jitter} {kin} {precision}: {info}", flush=True)
+    assert info["clusters"] > 0 and info["clustered_items"] > 0
+    r = rel_l2(out["cluster"][0], out["level"][0])
+    print(f"  cluster vs level: relative L2 {r:.3e}", flush=True)
+    assert r <= (1e-13 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kin,n,jitter", [("d3q4", 4, 0.15), ("d3q4", 6, 0.0), ("d3q6", 3, 0.1)])
+def test_cluster_sweep_parity_with_oracle(kin, n, jitter, monkeypatch):
+    """The cluster sweep against the oracle's assembled theta-scheme with
+    sparse LU: relative L2 <= 1e-12 on the full micro state."""
+    monkeypatch.setenv("GDN_SWEEP", "cluster")
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    path = kuhn_msh(n, jitter, 23)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    g, o = run_pair(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), kin, 0.3, 5.0)
+    r = rel_l2(g["wn"], o["wn"])
+    print(f"cluster sweep vs oracle {kin} {n}^3 +-{jitter}: {r:.3e}", flush=True)
+    assert r <= TOL
