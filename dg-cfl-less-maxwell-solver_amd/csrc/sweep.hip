@@ -1005,18 +1005,35 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 	__syncthreads();
 }
 
+/* the loads of the n items of a cluster whose descriptors are held one
+ * dword per lane */
+template <typename R>
+__device__ inline void cl_issue(int da, int db, int n, const R *__restrict__ fo,
+				const double *__restrict__ dinv, int ne, PrepLoads<R, 1> (&P)[CL_MAX])
+{
+#pragma unroll
+	for (int j = 0; j < CL_MAX; j++)
+		if (j < n) {
+			const ItemDesc d = cl_desc(da, db, j);
+			prep_issue<R, 1>(d.pad, d, fo, dinv, ne, 0, P[j]);
+		}
+}
+
 /*
  * The sweep over clusters in one launch. Cluster k (header chead[k]: first
  * item, n | nx << 8, inflow faces from outside) holds n <= CL_MAX items of
  * one velocity in level order; block b takes clusters b, b + G, ... in a
  * topological order of the cluster graphs, so that the earliest unfinished
  * cluster always has its inputs from outside ready (all G blocks are
- * co-resident; bounded waits report a timeout in *status). At the start of
- * a cluster the loads of all its items and a first poll of its nx inflow
- * faces from outside are issued together (one round trip); the items are
- * then solved in order (cl_item), taking the traces of in-cluster producers
- * from LDS (Wc) and the others from Xc once complete. Descriptors are read
- * one cluster ahead, headers two.
+ * co-resident; bounded waits report a timeout in *status). The loads of a
+ * cluster's items (PB) are issued while the block solves the cluster before
+ * it (PA) and moved to PA when that one is done, when they have long
+ * arrived. At the start of a cluster a first poll of its nx inflow faces
+ * from outside goes out before the next cluster's loads, so that its wait
+ * (loads complete in order) does not include them. The items are then
+ * solved in order (cl_item), taking the traces of in-cluster producers from
+ * LDS (Wc) and the others from Xc once complete. Descriptors are read two
+ * clusters ahead, headers three.
  */
 template <typename R>
 __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
@@ -1040,33 +1057,31 @@ __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
 	if (ci >= ncl)
 		return;
 	const int t = threadIdx.x;
+	PrepLoads<R, 1> PA[CL_MAX], PB[CL_MAX];
 	int4 h = chead[ci];
 	int da, db;
 	cl_load_desc(cdesc, h.x, h.y & 255, da, db);
-	int4 hn = make_int4(0, 0, 0, 0);
+	cl_issue<R>(da, db, uni(h.y & 255), fo, dinv, ne, PA);
+	int4 hn = ci + G < ncl ? chead[ci + G] : make_int4(0, 0, 0, 0);
+	int nda = 0, ndb = 0;
 	if (ci + G < ncl)
-		hn = chead[ci + G];
+		cl_load_desc(cdesc, uni(hn.x), uni(hn.y & 255), nda, ndb);
+	int4 hnn = ci + 2 * G < ncl ? chead[ci + 2 * G] : make_int4(0, 0, 0, 0);
 	for (; ci < ncl; ci += G) {
 		const int n = uni(h.y & 255), nx = uni((h.y >> 8) & 255);
 		const int hz = uni(h.z), hw = uni(h.w);
-		/* descriptors of the next cluster, header of the one after */
-		int nda = 0, ndb = 0;
-		if (ci + G < ncl)
-			cl_load_desc(cdesc, uni(hn.x), uni(hn.y & 255), nda, ndb);
-		int4 hnn = make_int4(0, 0, 0, 0);
-		if (ci + 2 * G < ncl)
-			hnn = chead[ci + 2 * G];
-		PrepLoads<R, 1> P[CL_MAX];
-#pragma unroll
-		for (int j = 0; j < CL_MAX; j++)
-			if (j < n) {
-				const ItemDesc d = cl_desc(da, db, j);
-				prep_issue<R, 1>(d.pad, d, fo, dinv, ne, 0, P[j]);
-			}
 		const int v = cl_dword(da, db, 9);
 		const unsigned all = (1u << nx) - 1u;
 		unsigned pend = cl_poll<R>(hz, hw, da, db, all, t < 36 ? all : 0u, ll, v, ne,
 					   epoch, Xc);
+		/* the next cluster's loads, its successor's descriptors, the header
+		 * after that */
+		if (ci + G < ncl)
+			cl_issue<R>(nda, ndb, uni(hn.y & 255), fo, dinv, ne, PB);
+		int nnda = 0, nndb = 0;
+		if (ci + 2 * G < ncl)
+			cl_load_desc(cdesc, uni(hnn.x), uni(hnn.y & 255), nnda, nndb);
+		const int4 hnnn = ci + 3 * G < ncl ? chead[ci + 3 * G] : make_int4(0, 0, 0, 0);
 #pragma unroll
 		for (int j = 0; j < CL_MAX; j++) {
 			if (j >= n)
@@ -1088,12 +1103,19 @@ __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
 				if (spin_expired(++spins, status))
 					break;
 			}
-			cl_item<R>(d, P[j], fo, xl, fn, ll, ne, neq, nv, epoch, T, W, Wc, Xc, &Wc[j][0]);
+			cl_item<R>(d, PA[j], fo, xl, fn, ll, ne, neq, nv, epoch, T, W, Wc, Xc,
+				   &Wc[j][0]);
 		}
+#pragma unroll
+		for (int j = 0; j < CL_MAX; j++)
+			PA[j] = PB[j];
 		h = hn;
 		hn = hnn;
+		hnn = hnnn;
 		da = nda;
 		db = ndb;
+		nda = nnda;
+		ndb = nndb;
 	}
 }
 
@@ -1123,12 +1145,12 @@ template <typename R> static int cluster_residency(gdh_ctx *c)
 	int dev = 0, ncu = 0, per_cu = 0;
 	HIPCHECK(hipGetDevice(&dev));
 	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_cluster<R>, 64, 0));
+	const auto kern = k_sweep_cluster<R>;
+	HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
-	k_sweep_cluster<R><<<g0, 64, 0, c->stream>>>(nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-						     nullptr, 0, 0, 0, nullptr, 0, nullptr, nullptr,
-						     disc, c->kp);
+	kern<<<g0, 64, 0, c->stream>>>(nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0,
+				       0, 0, nullptr, 0, nullptr, nullptr, disc, c->kp);
 	HIPCHECK(hipGetLastError());
 	std::vector<u64> h(2 * (size_t)g0);
 	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(), hipMemcpyDeviceToHost,
@@ -1310,17 +1332,17 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 		const ItemDesc *cdesc = c->cdesc;
 		int ncl = c->ncl;
 		const int cgrid = std::min(c->cl_grid, ncl);
+		const auto ck = k_sweep_cluster<R>;
 		if (edev) {
-			k_sweep_cluster<R><<<cgrid, 64, 0, c->stream>>>(
-				chead, cdesc, ncl, fo_c, fn, xl, dinv, nv, ne, neq, ll, 0u, edev,
-				status, disc, kp);
+			ck<<<cgrid, 64, 0, c->stream>>>(chead, cdesc, ncl, fo_c, fn, xl, dinv, nv, ne,
+							neq, ll, 0u, edev, status, disc, kp);
 			HIPCHECK(hipGetLastError());
 			return;
 		}
 		void *cargs[] = { &chead, &cdesc, &ncl, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq,
 				  &ll, &epoch, &edev, &status, &disc, &kp };
-		HIPCHECK(hipLaunchCooperativeKernel((const void *)k_sweep_cluster<R>, dim3(cgrid),
-						    dim3(64), cargs, 0, c->stream));
+		HIPCHECK(hipLaunchCooperativeKernel((const void *)ck, dim3(cgrid), dim3(64), cargs,
+						    0, c->stream));
 		return;
 	}
 	const int grid = std::min(c->sweep_grid, nitems);

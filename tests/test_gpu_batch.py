@@ -86,18 +86,25 @@ def run_batch(cases, kinetic, cfl, steps, precision, split=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kinetic,cases,cfl,fused,rk3",
-                         [(None, CASES_RK3, 0.5, "0", "pair"), (None, CASES_RK3, 0.5, "0", "tile"),
-                          (None, CASES_RK3, 0.5, "0", "soa"), ("d3q4", CASES_KIN, 5.0, "0", None),
-                          ("d3q4", CASES_KIN, 5.0, "1", None)])
-def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, rk3, monkeypatch):
+@pytest.mark.parametrize("kinetic,cases,cfl,fused,rk3,sweep",
+                         [(None, CASES_RK3, 0.5, "0", "pair", None),
+                          (None, CASES_RK3, 0.5, "0", "tile", None),
+                          (None, CASES_RK3, 0.5, "0", "soa", None),
+                          ("d3q4", CASES_KIN, 5.0, "0", None, "item"),
+                          ("d3q4", CASES_KIN, 5.0, "0", None, "cluster"),
+                          ("d3q4", CASES_KIN, 5.0, "1", None, "item")])
+def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, rk3, sweep,
+                                               monkeypatch):
     """Bit for bit, with the RK3 stage kernel (GDN_RK3: the three kernels
-    agree to rounding only) and the relaxation form (separate, or fused into
-    the sweep: GDN_RELAX_FUSED) fixed for the batch and the single
-    instances."""
+    agree to rounding only), the form of the sweep (GDN_SWEEP: item and
+    cluster sweeps agree to rounding only) and the relaxation form
+    (separate, or fused into the sweep: GDN_RELAX_FUSED) fixed for the batch
+    and the single instances."""
     monkeypatch.setenv("GDN_RELAX_FUSED", fused)
     if rk3:
         monkeypatch.setenv("GDN_RK3", rk3)
+    if sweep:
+        monkeypatch.setenv("GDN_SWEEP", sweep)
     print(f"batch {kinetic or 'rk3'} fused {fused} rk3 {rk3}", flush=True)
     steps = 7
     out, tiles = run_batch(cases, kinetic, cfl, steps, "fp64")
@@ -108,15 +115,18 @@ def test_fp64_batch_equals_each_instance_alone(kinetic, cases, cfl, fused, rk3, 
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kinetic,cases,cfl,fused",
-                         [(None, CASES_RK3, 0.5, "0"), ("d3q4", CASES_KIN, 5.0, "0"),
-                          ("d3q4", CASES_KIN, 5.0, "1")])
-def test_batch_graph_replay_split_steps(kinetic, cases, cfl, fused, monkeypatch):
+@pytest.mark.parametrize("kinetic,cases,cfl,fused,sweep",
+                         [(None, CASES_RK3, 0.5, "0", None), ("d3q4", CASES_KIN, 5.0, "0", "item"),
+                          ("d3q4", CASES_KIN, 5.0, "0", "cluster"),
+                          ("d3q4", CASES_KIN, 5.0, "1", "item")])
+def test_batch_graph_replay_split_steps(kinetic, cases, cfl, fused, sweep, monkeypatch):
     """7 steps in one call or as 3 + 1 + 2 + 1 (graph pairs of both parities,
     direct odd steps; with the fused relaxation the first step of the batch
     is taken directly so that every captured sweep relaxes its input) give
-    the same states bit for bit."""
+    the same states bit for bit (the sweep form fixed: GDN_SWEEP)."""
     monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+    if sweep:
+        monkeypatch.setenv("GDN_SWEEP", sweep)
     a, _ = run_batch(cases, kinetic, cfl, 7, "fp64")
     b, _ = run_batch(cases, kinetic, cfl, 7, "fp64", split=[3, 1, 2, 1])
     for x, y in zip(a, b):
