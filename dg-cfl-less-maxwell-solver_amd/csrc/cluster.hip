@@ -315,6 +315,13 @@ bool cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const std::v
 				}
 				code |= nib << (4 * f);
 			}
+			/* bit 16: some downstream neighbour lies outside the cluster,
+			 * so the item's values go to the hand-off words */
+			for (int f = 0; f < 4; f++) {
+				const int nb = nbr[4L * e + f];
+				if (nb >= 0 && cls[(4L * e + f) * nv + q.v] == 1 && jloc[nb] < 0)
+					code |= 1 << 16;
+			}
 			P.perm[pos + j] = ipos[(size_t)q.v * ne + e];
 			P.code[pos + j] = code;
 		}

@@ -251,8 +251,10 @@ struct ClusterPlan {
 				 * (byte x = 4 j + f: face f of item j) } */
 	std::vector<int> perm; /* cluster-order item -> level-order item */
 	std::vector<int> code; /* per item: 4 bits per face, external face x
-				* (0..7) or 8 + in-cluster producer j, at bit
-				* 12 + 4 f of the descriptor flags */
+				* (0..7) or 8 + in-cluster producer j, and bit
+				* 16 set when a downstream neighbour lies in
+				* another cluster; at bit 12 of the descriptor
+				* flags */
 	long multi; /* items in clusters of more than one element */
 	int levels; /* longest path of the cluster graphs */
 };

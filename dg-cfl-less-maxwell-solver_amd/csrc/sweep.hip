@@ -194,7 +194,9 @@ struct ItemDesc {
 		    * bit 8 + f: lagged inflow (cycle broken; value given in xl);
 		    * cluster order only, bits 12 + 4 f .. 15 + 4 f: where the
 		    * trace of face f comes from, inflow face x (0..7) of the
-		    * cluster from outside, or 8 + the in-cluster producer */
+		    * cluster from outside, or 8 + the in-cluster producer;
+		    * bit 28: a consumer lies in another cluster (the item's
+		    * values go to the hand-off words) */
 	int pad; /* cluster order only: the item's level-order index */
 	int nb[4]; /* neighbour element, or the boundary face index */
 	int map[4]; /* 4 bits per face node: the neighbour's local node */
@@ -929,8 +931,9 @@ __device__ inline void residency_probe(u64 *disc)
  * over every inflow face f in face order, u_f the ghost value, the lagged
  * value (1 - theta) f_nb + theta x_nb, the published trace of a producer
  * outside the cluster (Xc[x]) or of an earlier item of the cluster (Wc[j]).
- * Writes sol to fn, w = (1 - theta) f + theta sol to the hand-off words and,
- * rounded alike, to wl. Lanes t = 10 iw + i (t < 60).
+ * Writes sol to fn, w = (1 - theta) f + theta sol (rounded to R) to wl and,
+ * when a consumer lies in another cluster, to the hand-off words. Lanes
+ * t = 10 iw + i (t < 60).
  */
 template <typename R>
 __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
@@ -999,7 +1002,8 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 		const double sol = s - (double)fv;
 		fn[((long)d.v * ne + d.e) * 60 + t] = (R)sol;
 		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
-		Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
+		if (d.flags & (1 << 28))
+			Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
 		wl[t] = (double)wr;
 	}
 	__syncthreads();

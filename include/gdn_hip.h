@@ -196,7 +196,8 @@ void gdh_sweep_info(gdh_ctx *ctx, int out[5]);
  * the schedule of gdn_sweep_schedule. Fills head (4 ints per cluster: first
  * item, n | nx << 8, inflow faces from outside as bytes 4 j + f), perm
  * (cluster-order item -> level-order item, nv * ne) and code (per item, 4
- * bits per face: external face x < 8 or 8 + in-cluster producer);
+ * bits per face: external face x < 8 or 8 + in-cluster producer; bit 16:
+ * a downstream neighbour lies in another cluster);
  * stats = { levels of the cluster graphs, items in clusters of more than one
  * element }. Returns the number of clusters (at most nv * ne), or -1. */
 int gdh_cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const int *ranges,

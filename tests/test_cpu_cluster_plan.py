@@ -91,6 +91,8 @@ def check_plan(g, p, ranges=None):
     clus = np.repeat(np.arange(len(head)), n)
     where = np.empty(nv * ne, np.int64)  # cluster of item (v, e)
     where[ve] = clus
+    pos = np.empty(nv * ne, np.int64)  # cluster-order position of item (v, e)
+    pos[ve] = np.arange(nv * ne)
     inst = None if ranges is None else np.searchsorted(ranges, np.arange(ne), side="right") - 1
     for k in range(len(head)):
         i0 = head[k, 0]
@@ -115,7 +117,15 @@ def check_plan(g, p, ranges=None):
                 else:
                     assert (xb >> (8 * nib)) & 255 == 4 * j + f
                     assert where[v[0] * ne + u] < k
+                    # a producer read from outside its cluster publishes
+                    pu = pos[v[0] * ne + u]
+                    assert code[pu] >> 16 & 1
                     seen.append(nib)
+            # publish bit: some downstream neighbour lies in another cluster
+            out = [nbr[e[j], f] for f in range(4)
+                   if nbr[e[j], f] >= 0 and cls[e[j], f, v[0]] == 1]
+            ext = any(where[v[0] * ne + u] != k for u in out)
+            assert (code[i0 + j] >> 16 & 1) == int(ext)
         assert sorted(seen) == list(range(nx[k]))
 
 
