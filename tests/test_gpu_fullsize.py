@@ -1,0 +1,78 @@
+"""The relaxation workloads at their full sizes (Kuhn 32^3 x 6 = 196,608 and
+64^3 x 6 = 1,572,864 P2 tets), checked through properties that do not depend
+on the size: the two forms of the transport sweep (one item per workgroup;
+clusters of a grid cell solved by one wavefront) give the same state to
+rounding on the full 32^3 cavity (relative L2 <= 1e-13, FP64); the clock
+advances by exactly k dt; the L2 error against the exact cavity mode and
+against the exact plane wave stays at the level the method has on these
+meshes (bounds well above the measured 1.5e-4 .. 3e-4, far below an O(1)
+failure); and the binary32 form gives the binary64 form's L2 error to 1 %.
+The oracle itself is pinned at the smaller sizes (tests/test_gpu_relax.py).
+"""
+import numpy as np
+import pytest
+
+import gdon3d as G
+from helpers import rel_l2
+
+
+def cavity_32(steps, sweep, precision="fp64"):
+    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    mesh = G.Mesh(kuhn=(32, 0.0, 0))
+    sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic="d3q4"))
+    sim.set_time(2.0, 0.0, 5.0)
+    st = G.Stepper(sim, precision=precision)
+    info = G.sweep_info(st)
+    st.advance(steps)
+    st.sync()
+    t, err = st.t, st.error_l2()
+    st.finish()
+    wn = sim.wn()
+    dt = sim.s.dt
+    st.free()
+    sim.free()
+    return dict(wn=wn, t=t, err=err, dt=dt, info=info)
+
+
+@pytest.mark.gpu
+def test_cavity_32cubed_sweep_forms_agree(monkeypatch):
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    out = {}
+    for form in ("item", "cluster"):
+        monkeypatch.setenv("GDN_SWEEP", form)
+        out[form] = cavity_32(30, form)
+        assert out[form]["info"]["form"] == form
+        print(f"32^3 cavity, {form} sweep: t={out[form]['t']:.6f} L2 error {out[form]['err']:.4e}",
+              flush=True)
+    a, b = out["item"], out["cluster"]
+    assert a["t"] == b["t"] and abs(a["t"] - 30 * a["dt"]) <= 1e-12
+    r = rel_l2(b["wn"], a["wn"])
+    print(f"  item vs cluster: relative L2 {r:.3e}", flush=True)
+    assert r <= 1e-13
+    assert a["err"] < 1e-3 and abs(a["err"] - b["err"]) <= 1e-12 * a["err"]
+
+
+def plane_64(steps, precision):
+    kv = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    G.set_plane_wave(kv, (0.0, 0.0, 1.0))
+    mesh = G.Mesh(kuhn=(64, 0.1, 17))
+    sim = G.Simulation(mesh, G.Model("plane_wave", kinetic="d3q4"))
+    sim.set_time(1.0, 0.0, 5.0)
+    st = G.Stepper(sim, precision=precision)
+    form = G.sweep_info(st)["form"]
+    st.advance(steps)
+    st.sync()
+    err, t = st.error_l2(), st.t
+    st.free()
+    sim.free()
+    return err, t, form
+
+
+@pytest.mark.gpu
+def test_plane_wave_64cubed_precision_forms():
+    e64, t64, f64 = plane_64(20, "fp64")
+    e32, t32, f32 = plane_64(20, "fp32")
+    print(f"64^3 +-10% plane wave after 20 steps: FP64 {e64:.4e} ({f64} sweep), "
+          f"FP32 {e32:.4e} ({f32} sweep)", flush=True)
+    assert t64 == t32
+    assert e64 < 1e-3 and abs(e32 - e64) <= 0.01 * e64
