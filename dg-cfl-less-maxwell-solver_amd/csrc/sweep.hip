@@ -18,6 +18,7 @@ static void cap_setup(gdh_ctx *c);
 static void relax_choose(gdh_ctx *c);
 static void cluster_setup(gdh_ctx *c, const uint8_t *cls, const int *items_ve,
 			  const int *level_ptr, int nlevels);
+static int grid_cap(int g);
 
 /* own outward flux coefficient s_f = v . vn(e, f) */
 __device__ inline double face_coef(const double ct[9], int f, const double *v)
@@ -1204,7 +1205,7 @@ static void cluster_setup(gdh_ctx *c, const uint8_t *cls, const int *items_ve,
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	HIPCHECK(hipFree(perm));
 	HIPCHECK(hipFree(code));
-	c->cl_grid = c->fp32 ? cluster_residency<float>(c) : cluster_residency<double>(c);
+	c->cl_grid = grid_cap(c->fp32 ? cluster_residency<float>(c) : cluster_residency<double>(c));
 }
 
 /*
@@ -1278,6 +1279,16 @@ void sweep_setup(gdh_ctx *c)
 	const char *poll = getenv("GDN_SWEEP_POLL");
 	c->sweep_poll = poll ? !strcmp(poll, "full") : c->fp32;
 	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
+	c->sweep_grid = grid_cap(c->sweep_grid);
+}
+
+/* GDN_SWEEP_GRID=n: at most n workgroups in the one-launch sweeps (fewer
+ * than the resident count keeps every wait satisfiable) */
+static int grid_cap(int g)
+{
+	const char *env = getenv("GDN_SWEEP_GRID");
+	const int cap = env ? atoi(env) : 0;
+	return cap > 0 ? std::min(g, cap) : g;
 }
 
 __global__ void k_epoch_bump(unsigned *e)
