@@ -943,8 +943,12 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 			       const SweepTables &T, PrepScratch &W, const double (*Wc)[60],
 			       const double (*Xc)[36], double *wl)
 {
+	/* lanes 60..63 repeat lane 59's arithmetic (on its value, cl_issue
+	 * loads it for them) instead of branching around it; only stores are
+	 * limited to t < 60 */
 	const int t = lane_id();
-	const int iw = t / 10, i = t % 10;
+	const int tc = min(t, 59);
+	const int iw = tc / 10, i = tc % 10;
 	const R fv = P.fr[0];
 	W.D[t / 10][t % 10] = P.d0;
 	if (t < 36)
@@ -969,40 +973,35 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 				W.S[t] = u;
 			}
 			__syncthreads();
-			if (t < 60)
 #pragma unroll
-				for (int b = 0; b < 6; b++)
-					s += T.ML[f][i][b] * W.S[6 * iw + b];
+			for (int b = 0; b < 6; b++)
+				s += T.ML[f][i][b] * W.S[6 * iw + b];
 			__syncthreads();
 		} else {
 			const int code = (d.flags >> (12 + 4 * f)) & 15;
-			if (t < 60) {
-				if (code >= 8) {
-					const int m = d.map[f];
+			if (code >= 8) {
+				const int m = d.map[f];
 #pragma unroll
-					for (int b = 0; b < 6; b++)
-						s += T.ML[f][i][b] *
-						     Wc[code - 8][10 * iw + ((m >> (4 * b)) & 15)];
-				} else {
+				for (int b = 0; b < 6; b++)
+					s += T.ML[f][i][b] * Wc[code - 8][10 * iw + ((m >> (4 * b)) & 15)];
+			} else {
 #pragma unroll
-					for (int b = 0; b < 6; b++)
-						s += T.ML[f][i][b] * Xc[code][6 * iw + b];
-				}
+				for (int b = 0; b < 6; b++)
+					s += T.ML[f][i][b] * Xc[code][6 * iw + b];
 			}
 		}
 		y += cf * s;
 	}
-	if (t < 60)
-		W.S[t] = 2.0 * (double)fv - y;
+	W.S[tc] = 2.0 * (double)fv - y; /* lanes 59..63 write the same value */
 	__syncthreads();
-	if (t < 60) {
-		double s = 0.0;
+	double s = 0.0;
 #pragma unroll
-		for (int l = 0; l < 10; l++)
-			s += W.D[i][l] * W.S[10 * iw + l];
-		const double sol = s - (double)fv;
+	for (int l = 0; l < 10; l++)
+		s += W.D[i][l] * W.S[10 * iw + l];
+	const double sol = s - (double)fv;
+	const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
+	if (t < 60) {
 		fn[((long)d.v * ne + d.e) * 60 + t] = (R)sol;
-		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 		if (d.flags & (1 << 28))
 			Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
 		wl[t] = (double)wr;
@@ -1016,11 +1015,18 @@ template <typename R>
 __device__ inline void cl_issue(int da, int db, int n, const R *__restrict__ fo,
 				const double *__restrict__ dinv, int ne, PrepLoads<R, 1> (&P)[CL_MAX])
 {
+	const int t = lane_id();
 #pragma unroll
 	for (int j = 0; j < CL_MAX; j++)
 		if (j < n) {
+			/* every lane loads (lanes 60..63 lane 59's value, lanes 36..63
+			 * D^-1 entry 99 as their second entry), so that no load sits
+			 * behind a branch */
 			const ItemDesc d = cl_desc(da, db, j);
-			prep_issue<R, 1>(d.pad, d, fo, dinv, ne, 0, P[j]);
+			const double *D = dinv + 100L * d.pad;
+			P[j].fr[0] = fo[((long)d.v * ne + d.e) * 60 + min(t, 59)];
+			P[j].d0 = D[t];
+			P[j].d1 = D[64 + min(t, 35)];
 		}
 }
 
@@ -1102,7 +1108,8 @@ __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
 					need |= 1u << code;
 			}
 			int spins = 0;
-			for (unsigned um = cl_any(pend & need); um; um = cl_any(pend & need)) {
+			unsigned um = __ballot((pend & need) != 0u) ? cl_any(pend & need) : 0u;
+			for (; um; um = cl_any(pend & need)) {
 				__builtin_amdgcn_s_sleep(1);
 				pend = cl_poll<R>(hz, hw, da, db, um, pend, ll, v, ne, epoch, Xc);
 				if (spin_expired(++spins, status))
