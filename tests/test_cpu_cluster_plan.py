@@ -162,3 +162,14 @@ def test_cluster_plan_union_of_instances():
     p = plan(g, ranges)
     check_plan(g, p, np.asarray(ranges))
     assert len(p["head"]) == 4 * (27 + 64)
+
+
+@pytest.mark.parametrize("mesh,kinetic", [("t4_cube_8", "d3q4"), ("t4_cube_8", "d3q6"),
+                                          ("t4_cube_8_torus_128", "d3q4")])
+def test_cluster_plan_unstructured(mesh, kinetic):
+    """The reference's unstructured meshes (fixtures in tests/golden): most
+    clusters are split (grid cells cut across the flow), the invariants hold."""
+    import os
+    g = mesh_graph(os.path.join(os.path.dirname(__file__), "golden", mesh + ".msh"), kinetic)
+    p = plan(g)
+    check_plan(g, p)
