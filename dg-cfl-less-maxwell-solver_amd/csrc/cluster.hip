@@ -21,7 +21,10 @@
  * than CL_XMAX inflow faces from outside is cut, in level order, into
  * pieces that have at most CL_XMAX. Neither step can close a cycle: a
  * cycle through the result maps onto a closed walk through the clusters it
- * came from.
+ * came from. A single element left by these steps then joins the cluster
+ * of its producers when they all lie in one cluster with room (it brings no
+ * in-edge from outside, so no cycle can form): where grid cells cut across
+ * the flow (unstructured meshes) clusters then follow it.
  */
 #include "ctx.h"
 
@@ -242,6 +245,51 @@ bool cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const std::v
 			cid[e] = ncid - 1;
 			cnt++;
 			nxc += ex;
+		}
+		/* flow-following merge, in upwind-level order: a single-element
+		 * piece whose producers all lie in one piece of fewer than CL_MAX
+		 * elements joins it. Its in-edges then all come from inside that
+		 * piece, so a cycle through the merged piece would be one through
+		 * the piece and the single element before: none can form. This
+		 * recovers clusters where grid cells cut across the flow
+		 * (unstructured meshes). */
+		{
+			std::vector<int> psize(ncid, 0), lorder(ne);
+			for (int e = 0; e < ne; e++)
+				psize[cid[e]]++;
+			std::iota(lorder.begin(), lorder.end(), 0);
+			std::sort(lorder.begin(), lorder.end(), [&](int a, int b) {
+				const int la = lev[(size_t)v * ne + a], lb = lev[(size_t)v * ne + b];
+				return la != lb ? la < lb : a < b;
+			});
+			for (const int e : lorder) {
+				if (psize[cid[e]] != 1)
+					continue;
+				int c = -1, np = 0;
+				bool one = true;
+				for (int f = 0; f < 4; f++) {
+					const int u = dep(e, f);
+					if (u < 0)
+						continue;
+					np++;
+					if (c < 0)
+						c = cid[u];
+					else if (cid[u] != c)
+						one = false;
+				}
+				if (!one || np == 0 || psize[c] >= CL_MAX)
+					continue;
+				psize[cid[e]]--;
+				cid[e] = c;
+				psize[c]++;
+			}
+			/* members of each piece in upwind-level order */
+			std::sort(order.begin(), order.end(), [&](int a, int b) {
+				if (cid[a] != cid[b])
+					return cid[a] < cid[b];
+				const int la = lev[(size_t)v * ne + a], lb = lev[(size_t)v * ne + b];
+				return la != lb ? la < lb : a < b;
+			});
 		}
 		/* longest path over the pieces (Kahn); all placed = acyclic */
 		group_graph(ncid, ne, cid, dep, ap, aj);
