@@ -1001,7 +1001,7 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 	const double sol = s - (double)fv;
 	const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 	if (t < 60) {
-		fn[((long)d.v * ne + d.e) * 60 + t] = (R)sol;
+		__builtin_nontemporal_store((R)sol, fn + ((long)d.v * ne + d.e) * 60 + t);
 		if (d.flags & (1 << 28))
 			Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
 		wl[t] = (double)wr;
@@ -1021,12 +1021,13 @@ __device__ inline void cl_issue(int da, int db, int n, const R *__restrict__ fo,
 		if (j < n) {
 			/* every lane loads (lanes 60..63 lane 59's value, lanes 36..63
 			 * D^-1 entry 99 as their second entry), so that no load sits
-			 * behind a branch */
+			 * behind a branch; read once per sweep: non-temporal, which
+			 * leaves the caches to the hand-off words */
 			const ItemDesc d = cl_desc(da, db, j);
 			const double *D = dinv + 100L * d.pad;
-			P[j].fr[0] = fo[((long)d.v * ne + d.e) * 60 + min(t, 59)];
-			P[j].d0 = D[t];
-			P[j].d1 = D[64 + min(t, 35)];
+			P[j].fr[0] = __builtin_nontemporal_load(fo + ((long)d.v * ne + d.e) * 60 + min(t, 59));
+			P[j].d0 = __builtin_nontemporal_load(D + t);
+			P[j].d1 = __builtin_nontemporal_load(D + 64 + min(t, 35));
 		}
 }
 
