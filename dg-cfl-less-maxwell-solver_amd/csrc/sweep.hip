@@ -852,6 +852,28 @@ __device__ inline ItemDesc cl_desc(int a, int b, int j)
 	return d;
 }
 
+/* descriptor of item j of a cluster, read field by field where it is used
+ * (one readlane each, j a compile-time index once unrolled): the fields an
+ * item does not use are never read */
+struct DescLanes {
+	int a, b, j;
+	__device__ int dw(int k) const
+	{
+		const int x = 20 * j + k;
+		return x < 64 ? __builtin_amdgcn_readlane(a, x) : __builtin_amdgcn_readlane(b, x - 64);
+	}
+	__device__ double cf(int f) const
+	{
+		return __hiloint2double(dw(2 * f + 1), dw(2 * f));
+	}
+	__device__ int e() const { return dw(8); }
+	__device__ int v() const { return dw(9); }
+	__device__ int flags() const { return dw(10); }
+	__device__ int pad() const { return dw(11); }
+	__device__ int nb(int f) const { return dw(12 + f); }
+	__device__ int map(int f) const { return dw(16 + f); }
+};
+
 /* the descriptors of the n items from item i0 on, one dword per lane */
 __device__ inline void cl_load_desc(const ItemDesc *__restrict__ cdesc, int i0, int n, int &a,
 				    int &b)
@@ -937,7 +959,7 @@ __device__ inline void residency_probe(u64 *disc)
  * t = 10 iw + i (t < 60).
  */
 template <typename R>
-__device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
+__device__ inline void cl_item(const DescLanes &D, const PrepLoads<R, 1> &P,
 			       const R *__restrict__ fo, const R *__restrict__ xl,
 			       R *__restrict__ fn, u64 *ll, int ne, int neq, int nv, unsigned epoch,
 			       const SweepTables &T, PrepScratch &W, const double (*Wc)[60],
@@ -950,22 +972,27 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 	const int tc = min(t, 59);
 	const int iw = tc / 10, i = tc % 10;
 	const R fv = P.fr[0];
+	const int flags = D.flags();
 	W.D[t / 10][t % 10] = P.d0;
 	if (t < 36)
 		W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
 	double y = 0.0;
 #pragma unroll
 	for (int f = 0; f < 4; f++) {
-		if (!(d.flags & (1 << f)))
+		if (!(flags & (1 << f)))
 			continue;
-		const double cf = d.cf[f];
+		const double cf = D.cf(f);
 		double s = 0.0;
-		if (d.flags & (0x110 << f)) {
+		if (flags & (0x110 << f)) {
 			/* ghost or lagged: the trace staged through W.S */
 			if (t < 36) {
-				const long g = trace_index(d, f, t, ne, neq, nv);
+				const int q = t / 6, b = t % 6, v = D.v(), nbf = D.nb(f);
+				const long g = (flags & (16 << f))
+						       ? 60L * nv * ne + 6L * v * (neq - 10L * ne) + 36L * nbf + t
+						       : ((long)v * ne + nbf) * 60 + 10 * q +
+								 ((D.map(f) >> (4 * b)) & 15);
 				double u = (double)fo[g];
-				if (!(d.flags & (16 << f))) {
+				if (!(flags & (16 << f))) {
 					u = (1.0 - THETA) * u;
 					if (xl)
 						u += THETA * (double)xl[g];
@@ -978,9 +1005,9 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 				s += T.ML[f][i][b] * W.S[6 * iw + b];
 			__syncthreads();
 		} else {
-			const int code = (d.flags >> (12 + 4 * f)) & 15;
+			const int code = (flags >> (12 + 4 * f)) & 15;
 			if (code >= 8) {
-				const int m = d.map[f];
+				const int m = D.map(f);
 #pragma unroll
 				for (int b = 0; b < 6; b++)
 					s += T.ML[f][i][b] * Wc[code - 8][10 * iw + ((m >> (4 * b)) & 15)];
@@ -1001,9 +1028,10 @@ __device__ inline void cl_item(const ItemDesc &d, const PrepLoads<R, 1> &P,
 	const double sol = s - (double)fv;
 	const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 	if (t < 60) {
-		__builtin_nontemporal_store((R)sol, fn + ((long)d.v * ne + d.e) * 60 + t);
-		if (d.flags & (1 << 28))
-			Handoff<R>::put(ll_item<R>(ll, d.v, d.e, ne), t, wr, epoch);
+		const int v = D.v(), e = D.e();
+		__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
+		if (flags & (1 << 28))
+			Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 		wl[t] = (double)wr;
 	}
 	__syncthreads();
@@ -1023,9 +1051,9 @@ __device__ inline void cl_issue(int da, int db, int n, const R *__restrict__ fo,
 			 * D^-1 entry 99 as their second entry), so that no load sits
 			 * behind a branch; read once per sweep: non-temporal, which
 			 * leaves the caches to the hand-off words */
-			const ItemDesc d = cl_desc(da, db, j);
-			const double *D = dinv + 100L * d.pad;
-			P[j].fr[0] = __builtin_nontemporal_load(fo + ((long)d.v * ne + d.e) * 60 + min(t, 59));
+			const DescLanes d{ da, db, j };
+			const double *D = dinv + 100L * d.pad();
+			P[j].fr[0] = __builtin_nontemporal_load(fo + ((long)d.v() * ne + d.e()) * 60 + min(t, 59));
 			P[j].d0 = __builtin_nontemporal_load(D + t);
 			P[j].d1 = __builtin_nontemporal_load(D + 64 + min(t, 35));
 		}
@@ -1098,13 +1126,14 @@ __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
 		for (int j = 0; j < CL_MAX; j++) {
 			if (j >= n)
 				break;
-			const ItemDesc d = cl_desc(da, db, j);
+			const DescLanes d{ da, db, j };
 			/* inflow faces from outside that this item reads */
 			unsigned need = 0;
-			const int sf = slot_faces(d);
+			const int flags = d.flags();
+			const int sf = flags & ~(flags >> 4) & ~(flags >> 8) & 15;
 #pragma unroll
 			for (int f = 0; f < 4; f++) {
-				const int code = (d.flags >> (12 + 4 * f)) & 15;
+				const int code = (flags >> (12 + 4 * f)) & 15;
 				if ((sf & (1 << f)) && code < 8)
 					need |= 1u << code;
 			}
