@@ -163,3 +163,19 @@ def test_rk3_permittivity_plane_wave():
     print(f"  errors eps=1 {errs[1.0]:.3e} eps=4 {errs[4.0]:.3e}", flush=True)
     assert errs[4.0] < 5e-3 and errs[1.0] < 5e-3
     assert errs[4.0] < 3 * errs[1.0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sweep", ["item", "cluster"])
+def test_batch_with_one_cube_instance(sweep, monkeypatch):
+    """Ragged batch: an instance of a single Kuhn cube (6 tets) between two
+    larger ones; each instance equals its own stepper run bit for bit."""
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    monkeypatch.setenv("GDN_SWEEP", sweep)
+    cases = [((3, 0.1, 1), cavity((1, 1, 1), (1, -1, 0))),
+             ((1, 0.0, 0), cavity((1, 1, 2), (1, 0, -1))),
+             ((2, 0.0, 0), cavity((2, 1, 1), (0, 1, -1)))]
+    out, _ = run_batch(cases, "d3q4", 5.0, 5, "fp64")
+    for case, wn in zip(cases, out):
+        ref = single(case, "d3q4", 5.0, 5)
+        assert np.array_equal(wn, ref), rel_l2(wn, ref)

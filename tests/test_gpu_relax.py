@@ -213,3 +213,19 @@ def test_cluster_sweep_unstructured_meshes(mesh, kin, imposed, monkeypatch):
     r = rel_l2(g["wn"], o["wn"])
     print(f"cluster sweep vs oracle on {mesh} {kin}: {r:.3e}", flush=True)
     assert r <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 2])
+@pytest.mark.parametrize("form", ["level", "item", "cluster"])
+def test_smallest_meshes_every_sweep_form(n, form, monkeypatch):
+    """Edge sizes: one Kuhn cube (6 tets, one cluster per velocity, every
+    element next to the boundary) and 2^3 cubes, every form of the sweep
+    against the oracle: relative L2 <= 1e-12."""
+    monkeypatch.setenv("GDN_SWEEP", form)
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    path = kuhn_msh(n, 0.0, 0)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    g, o = run_pair(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", 0.5, 5.0)
+    assert g["itermax"] == o["itermax"]
+    assert rel_l2(g["wn"], o["wn"]) <= TOL
