@@ -47,8 +47,8 @@ SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
-    ("fp64", "k_rk3_pair"): (1317320.8 / 0.5 + 1474560.0) * 1024 / 1572864,
-    ("fp32", "k_rk3_tile"): (837410.6 / 0.5 + 737280.0) * 1024 / 1572864,
+    ("fp64", "k_rk3_pair"): (1127196.9 / 0.5 + 1474591.1) * 1024 / 1572864,
+    ("fp32", "k_rk3_tile"): (746906.2 / 0.5 + 737280.0) * 1024 / 1572864,
 }
 
 

@@ -283,9 +283,9 @@ __global__ __launch_bounds__(BS) void k_rk3_soa(
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
 			const T d = acc[q][i] * (q < 3 ? idE : invdet);
-			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
-			K2[g] = k;
-			Wo[g] = Wr[g] + BcT * k;
+			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
+			__builtin_nontemporal_store(k, K2 + g);
+			__builtin_nontemporal_store(Wr[g] + BcT * k, Wo + g);
 		}
 }
 
@@ -370,9 +370,9 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
 			const T d = acc[q][i] * (q < 3 ? idE : invdet);
-			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
-			K2[g] = k;
-			Wo[g] = Ws[(q * 10 + i) * 64 + t] + BcT * k;
+			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
+			__builtin_nontemporal_store(k, K2 + g);
+			__builtin_nontemporal_store(Ws[(q * 10 + i) * 64 + t] + BcT * k, Wo + g);
 		}
 }
 
@@ -587,9 +587,9 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 			const int r = (3 * h + p) * 10 + i;
 			const long g = r * ne + e;
 			const T d = acc[p][i] * sc;
-			const T k = first ? -dtT * d : AcT * K2[g] - dtT * d;
-			K2[g] = k;
-			Wo[g] = Ws[r * 32 + c] + BcT * k;
+			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
+			__builtin_nontemporal_store(k, K2 + g);
+			__builtin_nontemporal_store(Ws[r * 32 + c] + BcT * k, Wo + g);
 		}
 }
 
