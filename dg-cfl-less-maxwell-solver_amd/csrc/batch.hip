@@ -14,7 +14,7 @@ using namespace gdh;
 void ghost_fill_batch(gdh_ctx *c, double cstage);
 double error_range(gdh_ctx *c, int e0, int e1, double t, const ImposedPar &ip,
 		   const double *hv);
-void rk3_trial(gdh_ctx *c);
+void rk3_trial(gdh_ctx *c, int first);
 
 __global__ void k_time_bump(double *__restrict__ t, const double *__restrict__ dt, int n)
 {
@@ -142,7 +142,7 @@ extern "C" void gdh_batch_autotune(gdh_ctx *c, int out[2])
 					continue;
 				c->rk3_bs = cand[k][0];
 				c->rk3_tile = cand[k][1];
-				ms[k] = std::min(ms[k], time_launches(c, 5, [&] { rk3_trial(c); }));
+				ms[k] = std::min(ms[k], time_launches(c, 5, [&] { rk3_trial(c, 1); }));
 			}
 		int pick = -1;
 		for (int k = 0; k < 4; k++)
@@ -152,6 +152,20 @@ extern "C" void gdh_batch_autotune(gdh_ctx *c, int out[2])
 		c->rk3_tile = cand[pick][1];
 		if (c->rk3_tile == 2)
 			c->rk3_bs = 32; /* elements per block of the pair kernel */
+		/* the k2 stream with non-temporal or ordinary loads and stores,
+		 * timed on later-stage launches (the ones that read k2) */
+		float mt[2] = { 1e30f, 1e30f };
+		for (int round = 0; round < 2; round++)
+			for (int k = 0; k < 2; k++) {
+				c->rk3_ntk = 1 - k;
+				mt[k] = std::min(mt[k], time_launches(c, 5, [&] { rk3_trial(c, 0); }));
+			}
+		c->rk3_ntk = mt[0] <= mt[1];
+		if (getenv("GDN_VERBOSE"))
+			fprintf(stderr, "[gdh] batch RK3: %s kernel, %d elements per block; k2 stream "
+				"non-temporal %.3f ms, ordinary %.3f ms per stage: %s\n",
+				c->rk3_tile == 2 ? "pair" : c->rk3_tile ? "tile" : "soa", c->rk3_bs,
+				mt[0], mt[1], c->rk3_ntk ? "non-temporal" : "ordinary");
 	} else {
 		static const int cand[3] = { 128, 256, 512 };
 		float ms[3] = { 1e30f, 1e30f, 1e30f };

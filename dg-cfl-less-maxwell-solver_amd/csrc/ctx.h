@@ -100,6 +100,8 @@ struct gdh_ctx {
 	int rk3_bs, relax_bs; /* tile sizes (autotuned for batches) */
 	int rk3_tile; /* RK3 stage: 0 global state, 1 LDS tile, 2 LDS tile with
 		       * two lanes per element */
+	int rk3_ntk; /* RK3 stage: k2 with non-temporal loads and stores (1, the
+		      * default; batches time both) or ordinary ones (0) */
 	long nlagged; /* inflow faces lagged to break upwind cycles */
 	void *wx; /* zero state (set-up of the capacitance matrices) */
 	void *wz; /* lagged values z, scattered into a state-shaped buffer */

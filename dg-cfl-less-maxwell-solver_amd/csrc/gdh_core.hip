@@ -224,6 +224,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	}
 	c->graph[0] = c->graph[1] = nullptr;
 	c->rk3_bs = 128;
+	c->rk3_ntk = 1;
 	{
 		/* RK3 stage kernel, all three over the same arithmetic: the tile's
 		 * state staged in LDS with two lanes per element ("pair", the binary64

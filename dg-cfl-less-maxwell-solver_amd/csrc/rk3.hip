@@ -54,6 +54,26 @@ template <typename T> __device__ __forceinline__ const T *reload_ptr(const T *p)
 	return p;
 }
 
+/* The stage's k2 stream (read once and written once per stage): with
+ * non-temporal hints (NT) or ordinary loads and stores. Which is faster
+ * depends on the workload (measured: non-temporal at 64^3, ordinary on
+ * the batch of small instances in binary32), so both are built. */
+template <bool NT, typename T> __device__ __forceinline__ T k2_load(const T *p)
+{
+	if constexpr (NT)
+		return __builtin_nontemporal_load(p);
+	else
+		return *p;
+}
+
+template <bool NT, typename T> __device__ __forceinline__ void k2_store(T v, T *p)
+{
+	if constexpr (NT)
+		__builtin_nontemporal_store(v, p);
+	else
+		*p = v;
+}
+
 /* Materialise the accumulators at a phase boundary: without it the
  * optimiser re-associates the sums across phases and sinks every product to
  * the final update, keeping all intermediate values live. */
@@ -223,7 +243,7 @@ __device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
  * the flux/volume arithmetic, geometry G always FP64; BS elements per block.
  * Batches: the time step of the column's instance (dt_tab[inst[e]]) and,
  * with ieps, dE/dt scaled by 1/eps of the column (eps dE/dt = curl H). */
-template <typename T, int BS>
+template <typename T, int BS, bool NTK>
 __global__ __launch_bounds__(BS) void k_rk3_soa(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
@@ -283,8 +303,8 @@ __global__ __launch_bounds__(BS) void k_rk3_soa(
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
 			const T d = acc[q][i] * (q < 3 ? idE : invdet);
-			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
-			__builtin_nontemporal_store(k, K2 + g);
+			const T k = first ? -dtT * d : AcT * k2_load<NTK>(K2 + g) - dtT * d;
+			k2_store<NTK>(k, K2 + g);
 			__builtin_nontemporal_store(Wr[g] + BcT * k, Wo + g);
 		}
 }
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(BS) void k_rk3_soa(
  * come from global memory. One 64-lane block per tile, one element per
  * lane; the arithmetic is that of k_rk3_soa.
  */
-template <typename T>
+template <typename T, bool NTK>
 __global__ __launch_bounds__(64) void k_rk3_tile(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
@@ -370,8 +390,8 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 		for (int i = 0; i < 10; i++) {
 			const long g = (q * 10 + i) * ne + e;
 			const T d = acc[q][i] * (q < 3 ? idE : invdet);
-			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
-			__builtin_nontemporal_store(k, K2 + g);
+			const T k = first ? -dtT * d : AcT * k2_load<NTK>(K2 + g) - dtT * d;
+			k2_store<NTK>(k, K2 + g);
 			__builtin_nontemporal_store(Ws[(q * 10 + i) * 64 + t] + BcT * k, Wo + g);
 		}
 }
@@ -521,7 +541,7 @@ __device__ __forceinline__ void face_pair(T (&acc)[3][10], const T *Ws, const T 
 	}
 }
 
-template <typename T>
+template <typename T, bool NTK>
 __global__ __launch_bounds__(64) void k_rk3_pair(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
@@ -587,29 +607,29 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 			const int r = (3 * h + p) * 10 + i;
 			const long g = r * ne + e;
 			const T d = acc[p][i] * sc;
-			const T k = first ? -dtT * d : AcT * __builtin_nontemporal_load(K2 + g) - dtT * d;
-			__builtin_nontemporal_store(k, K2 + g);
+			const T k = first ? -dtT * d : AcT * k2_load<NTK>(K2 + g) - dtT * d;
+			k2_store<NTK>(k, K2 + g);
 			__builtin_nontemporal_store(Ws[r * 32 + c] + BcT * k, Wo + g);
 		}
 }
 
-template <typename T, int BS>
+template <typename T, int BS, bool NTK>
 static void rk3_launch_bs(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 			  double dt, int first, const void *W, void *Wo)
 {
-	k_rk3_soa<T, BS><<<nblocks(c->ne, BS), BS, 0, c->stream>>>(
+	k_rk3_soa<T, BS, NTK><<<nblocks(c->ne, BS), BS, 0, c->stream>>>(
 		(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
 		c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
 		c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
 	HIPCHECK(hipGetLastError());
 }
 
-template <typename T>
-static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
-		       double dt, int first, const void *W, void *Wo)
+template <typename T, bool NTK>
+static void rk3_launch_nt(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+			  double dt, int first, const void *W, void *Wo)
 {
 	if (c->rk3_tile == 2) {
-		k_rk3_pair<T><<<nblocks(c->ne, 32), 64, 0, c->stream>>>(
+		k_rk3_pair<T, NTK><<<nblocks(c->ne, 32), 64, 0, c->stream>>>(
 			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
 			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
 			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
@@ -617,7 +637,7 @@ static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc
 		return;
 	}
 	if (c->rk3_tile) {
-		k_rk3_tile<T><<<nblocks(c->ne, 64), 64, 0, c->stream>>>(
+		k_rk3_tile<T, NTK><<<nblocks(c->ne, 64), 64, 0, c->stream>>>(
 			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
 			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
 			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
@@ -625,10 +645,20 @@ static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc
 		return;
 	}
 	switch (c->rk3_bs) {
-	case 64: rk3_launch_bs<T, 64>(c, tab, Ac, Bc, dt, first, W, Wo); break;
-	case 256: rk3_launch_bs<T, 256>(c, tab, Ac, Bc, dt, first, W, Wo); break;
-	default: rk3_launch_bs<T, 128>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	case 64: rk3_launch_bs<T, 64, NTK>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	case 256: rk3_launch_bs<T, 256, NTK>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+	default: rk3_launch_bs<T, 128, NTK>(c, tab, Ac, Bc, dt, first, W, Wo); break;
 	}
+}
+
+template <typename T>
+static void rk3_launch(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+		       double dt, int first, const void *W, void *Wo)
+{
+	if (c->rk3_ntk)
+		rk3_launch_nt<T, true>(c, tab, Ac, Bc, dt, first, W, Wo);
+	else
+		rk3_launch_nt<T, false>(c, tab, Ac, Bc, dt, first, W, Wo);
 }
 
 static void rk3_tables_init()
@@ -648,16 +678,20 @@ static void rk3_tables_init()
 			   hipMemcpyHostToDevice));
 }
 
-/* One first-stage launch from the current state into the other buffer
- * (timing only: used by the tile autotuner before a run starts). */
-void rk3_trial(gdh_ctx *c)
+/* One stage launch from the current state into the other buffer (timing
+ * only: used by the tile autotuner before a run starts). A first stage does
+ * not read k2; a later one (first = 0) does. Both overwrite only the other
+ * state buffer and k2, which the run's first stage writes before it reads
+ * them. */
+void rk3_trial(gdh_ctx *c, int first)
 {
 	rk3_tables_init();
+	const double A = first ? 0.0 : -5.0 / 9.0;
 	if (c->fp32)
-		rk3_launch<float>(c, d_rk3_tables32, 0.0, 1.0 / 3.0, 1e-3, 1,
+		rk3_launch<float>(c, d_rk3_tables32, A, 1.0 / 3.0, 1e-3, first,
 				  c->w[c->cur], c->w[1 - c->cur]);
 	else
-		rk3_launch<double>(c, d_rk3_tables64, 0.0, 1.0 / 3.0, 1e-3, 1,
+		rk3_launch<double>(c, d_rk3_tables64, A, 1.0 / 3.0, 1e-3, first,
 				   c->w[c->cur], c->w[1 - c->cur]);
 }
 
