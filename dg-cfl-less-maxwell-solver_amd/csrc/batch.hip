@@ -153,15 +153,18 @@ extern "C" void gdh_batch_autotune(gdh_ctx *c, int out[2])
 		if (c->rk3_tile == 2)
 			c->rk3_bs = 32; /* elements per block of the pair kernel */
 		/* the k2 stream with non-temporal or ordinary loads and stores,
-		 * timed on later-stage launches (the ones that read k2) */
+		 * timed on later-stage launches (the ones that read k2), unless
+		 * GDN_RK3_K2 fixes it */
 		float mt[2] = { 1e30f, 1e30f };
-		for (int round = 0; round < 2; round++)
+		const int fixed = getenv("GDN_RK3_K2") != nullptr;
+		for (int round = 0; round < 2 && !fixed; round++)
 			for (int k = 0; k < 2; k++) {
 				c->rk3_ntk = 1 - k;
 				mt[k] = std::min(mt[k], time_launches(c, 5, [&] { rk3_trial(c, 0); }));
 			}
-		c->rk3_ntk = mt[0] <= mt[1];
-		if (getenv("GDN_VERBOSE"))
+		if (!fixed)
+			c->rk3_ntk = mt[0] <= mt[1];
+		if (getenv("GDN_VERBOSE") && !fixed)
 			fprintf(stderr, "[gdh] batch RK3: %s kernel, %d elements per block; k2 stream "
 				"non-temporal %.3f ms, ordinary %.3f ms per stage: %s\n",
 				c->rk3_tile == 2 ? "pair" : c->rk3_tile ? "tile" : "soa", c->rk3_bs,

@@ -224,7 +224,12 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	}
 	c->graph[0] = c->graph[1] = nullptr;
 	c->rk3_bs = 128;
-	c->rk3_ntk = 1;
+	{
+		/* RK3 stage k2 stream: non-temporal (default) or ordinary loads and
+		 * stores; GDN_RK3_K2=nt|plain overrides (batches then skip timing it) */
+		const char *k = getenv("GDN_RK3_K2");
+		c->rk3_ntk = !(k && !strcmp(k, "plain"));
+	}
 	{
 		/* RK3 stage kernel, all three over the same arithmetic: the tile's
 		 * state staged in LDS with two lanes per element ("pair", the binary64

@@ -138,3 +138,33 @@ def test_rk3_tiled_stage_equals_global_stage(precision, boundary, monkeypatch):
         d = rel_l2(out[k], out["soa"])
         print(f"{k} vs global RK3 stage, {precision} {boundary}: rel L2 {d:.3e}", flush=True)
         assert d <= (1e-13 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_rk3_k2_stream_hints_bit_identical(precision, monkeypatch):
+    """The RK3 stage kernels load and store k2 with non-temporal hints
+    (GDN_RK3_K2=nt, the default for one mesh) or ordinary accesses
+    (GDN_RK3_K2=plain, which the batch autotuner may pick): a cache hint
+    changes no value, so every stage kernel gives the same state bit for bit
+    after a whole solve (6^3 x 6 tets, jittered, upwind boundary)."""
+    path = kuhn_msh(6, 0.1, 3)
+    G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
+    for kern in ("soa", "tile", "pair"):
+        monkeypatch.setenv("GDN_RK3", kern)
+        out = {}
+        for hint in ("nt", "plain"):
+            monkeypatch.setenv("GDN_RK3_K2", hint)
+            mesh = G.Mesh(path=path)
+            sim = G.Simulation(mesh, G.Model("plane_wave"))
+            sim.set_time(0.1, 0.0, 0.5)
+            st = G.Stepper(sim, precision=precision)
+            st.advance(sim.s.itermax)
+            st.sync()
+            st.finish()
+            out[hint] = sim.wn()
+            st.free()
+            sim.free()
+        print(f"{kern} {precision}: max |nt - plain| = "
+              f"{np.abs(out['nt'] - out['plain']).max():.3e}", flush=True)
+        assert np.array_equal(out["nt"], out["plain"])
