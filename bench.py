@@ -139,9 +139,18 @@ def bench_rk3(args):
                                  in RK3_TRAFFIC_BYTES_PER_ELEM_STAGE else None),
                      "kernel": kernel,
                      "bytes_per_element_stage": bytes_elem,
-                     "avg_launch_us": stage_s * 1e6},
+                     "avg_launch_us": stage_s * 1e6,
+                     "copy_peak_measured": copy_peak_gbs()},
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_peak_gbs(reps=20):
+    """Measured device copy rate in GB/s (1 GiB read + 1 GiB written per
+    launch, kernel k_copy_calib), taken after the timed region: the practical
+    HBM ceiling reported beside the 8 TB/s specification."""
+    ms = G.load_library().gdh_copy_calibration(1 << 30, reps)
+    return 2 * (1 << 30) / ms / 1e6 if ms > 0 else None
 
 
 def copy_calibration(args):
@@ -263,7 +272,8 @@ def bench_batch(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "whole step (all kernels, graph replay)",
-                     "bytes_per_dof_update": b_alg},
+                     "bytes_per_dof_update": b_alg,
+                     "copy_peak_measured": copy_peak_gbs()},
     }
     print(json.dumps(line), flush=True)
 
@@ -490,6 +500,7 @@ def main():
             "bytes_per_element_step": sweep_bytes,
             "launches_per_step": launches,
             "avg_launch_us": sweep_s / max(launches, 1) * 1e6,
+            "copy_peak_measured": copy_peak_gbs(),
         },
     }
     if not args.no_cpu_baseline and default_workload:

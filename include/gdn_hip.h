@@ -270,8 +270,8 @@ void gdh_learned_solve_stats(gdh_ctx *ctx, gdh_mlp *mlp, int n, const int *idx,
 			     double tol, int maxit, double *out, int *iters);
 
 /* Copies `bytes` (read) into a second buffer (written) `reps` times with a
- * plain streaming kernel (k_copy_calib) after one warm-up launch; returns
- * the mean milliseconds per launch. No counterpart in the reference: the
+ * plain streaming kernel (k_copy_calib) after one warm-up launch, at 8, 16,
+ * 32 and 64 blocks per CU; returns the fastest mean milliseconds per launch. No counterpart in the reference: the
  * benchmark uses it for a measured copy bandwidth and to calibrate the
  * profiler's HBM byte counters against a kernel of known traffic. */
 double gdh_copy_calibration(size_t bytes, int reps);
