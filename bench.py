@@ -11,13 +11,19 @@ node-local relaxation.
 
 value = DoF-updates/s = (24 * 10 * nelem) * K / (wall time of K steps), with
 the state resident in HBM (no host transfer inside the timed region).
-roofline: the transport sweep (dominant kernel), algorithmic bytes per
-element-step = 480 FP64 values read/written (3840 B) + 112 B geometry + 16 B
-level indices = 3968 B, divided by the sweep's device time (HIP events on
-the solver stream), against 8 TB/s.
+roofline: the transport sweep (dominant kernel, one launch per step).
+Algorithmic bytes per DoF-update (s = 8 FP64, 4 FP32): transport reads and
+writes the 24 micro-fields (2 s), the relaxation reads and writes them (2 s),
+plus 112 B geometry + 16 B level indices per element (128 / 240 per DoF):
+4 s + 0.53 = 32.5 B (FP64), 16.5 B (FP32). When the relaxation is fused into
+the sweep, one sweep launch does all of it, (960 s + 128) B per element; when
+it runs as a pass of its own, the sweep launch does the transport half,
+(480 s + 128) B per element. achieved = those bytes per launch / the sweep's
+mean launch time (HIP events on the solver stream), against 8 TB/s;
+achieved_step = 32.5 B (16.5 B) x DoF-updates/s over the whole step.
 cpu_baseline: the oracle's restatement of the reference CPU algorithm
 (assembled operators, sparse LU per velocity) on a bounded sample of the same
-workload: the first few steps at 32^3, assembly excluded.
+workload: the first steps at 32^3, assembly excluded.
 """
 import argparse
 import json
@@ -52,6 +58,20 @@ RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
 }
 
 
+def host_cpu():
+    """(logical CPUs of this host, CPU model name from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
 def cpu_baseline(n, steps):
     """Oracle (C99 + OpenMP restatement of the reference) on host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -71,12 +91,14 @@ def cpu_baseline(n, steps):
     s.set_time(2.0, 0.0, 5.0)
     s.solve(steps)
     dof = 24 * 10 * om.nelem
+    nproc, model = host_cpu()
     out = dict(value=dof * steps / s.t_run, unit="DoF-updates/s", cores=threads,
-               kind="port",
+               kind="port", host_nproc=nproc, host_cpu_model=model,
                sample=f"oracle relaxation D3Q4 FP64 on Kuhn {n}^3x6, first {steps} "
                       f"steps timed ({s.t_run:.2f} s; assembly {s.t_asm:.2f} s excluded); "
                       f"transport parallel over the 4 velocities, relaxation serial, "
-                      f"as the reference")
+                      f"as the reference; {threads} of {nproc} host threads ({model}); "
+                      f"the KLU variant is not timed: SuiteSparse is not installed")
     s.free()
     om.free()
     return out
@@ -358,8 +380,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--energy-every", type=int, default=10,
+                    help="relax: energy history sampled every k steps over a second, "
+                         "untimed run to t_end (0: none)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp32: the precision form (binary32 fields)")
     ap.add_argument("--jitter", type=float, default=0.0,
@@ -449,15 +474,35 @@ def main():
     relax_s = prof[1] / args.profile_steps
     launches = prof[3] / args.profile_steps
     s_val = 4 if args.precision == "fp32" else 8
-    sweep_bytes = 480 * s_val + 112 + 16
-    achieved = sweep_bytes * nelem / sweep_s / 1e9
+    # algorithmic bytes of one sweep launch: transport and relaxation when the
+    # relaxation is fused into the sweep, the transport half otherwise
+    transport_bytes = 480 * s_val + 112 + 16
+    sweep_bytes = 960 * s_val + 128 if relax_fused else transport_bytes
+    launch_s = sweep_s / max(launches, 1)
+    achieved = sweep_bytes * nelem / max(launches, 1) / launch_s / 1e9
     print(f"[bench] per step: sweep {sweep_s*1e3:.3f} ms ({launches:.0f} launches), "
           f"relax {relax_s*1e3:.3f} ms, ghosts {prof[2]/args.profile_steps*1e3:.3f} ms",
           file=sys.stderr, flush=True)
+    # energy history, untimed: the whole run from t = 0 to t_end again, the
+    # discrete energy 1/2 w^T M w (binary64 device reduction) after every step
+    hist_t, hist_e = [], []
+    if args.energy_every > 0:
+        G.load_library().dg_stepper_init(st.h)
+        hist_t.append(0.0)
+        hist_e.append(st.energy())
+        for k in range(1, sim_itermax + 1):
+            st.advance(1)
+            if k % args.energy_every == 0 or k == sim_itermax:
+                hist_t.append(st.t)
+                hist_e.append(st.energy())
+        print(f"[bench] energy history: {len(hist_e)} samples, E(0) = {hist_e[0]:.10e}, "
+              f"E(t_end = {hist_t[-1]:.5f}) = {hist_e[-1]:.10e} (ratio {hist_e[-1]/hist_e[0]:.8f})",
+              file=sys.stderr, flush=True)
     st.free()
     sim.free()
 
     value = dof * args.steps / elapsed
+    b_alg = 4 * s_val + 128 / 240
     line = {
         "metric": METRIC,
         "value": value,
@@ -482,10 +527,16 @@ def main():
             "t_final": t_final,
             "l2_error": err,
             "hbm_gb_s_sweep": achieved,
+            "sweep_ms": sweep_s * 1e3,
+            "relaxation_pass_ms": relax_s * 1e3,
             "relaxation_fused_into_sweep": bool(relax_fused),
             "sweep_form": sweep["form"],
             "sweep_clusters": sweep["clusters"],
             "sweep_resident_workgroups": sweep["grid"],
+            "energy_history": ({"t": hist_t, "energy": hist_e,
+                                "ratio_end": hist_e[-1] / hist_e[0],
+                                "note": "1/2 w^T M w, binary64 device reduction, second "
+                                        "untimed run from t = 0"} if hist_e else None),
         },
         "roofline": {
             "bound": "hbm",
@@ -496,10 +547,16 @@ def main():
             "traffic": (traffic * nelem if default_workload and traffic else None),
             "traffic_gb_s": (traffic * nelem / sweep_s / 1e9
                              if default_workload and traffic else None),
-            "kernel": f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step)",
-            "bytes_per_element_step": sweep_bytes,
+            "kernel": f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step"
+                      f"{', relaxation fused' if relax_fused else ''})",
+            "bytes_per_element_launch": sweep_bytes,
+            "bytes_per_dof_update": sweep_bytes / 240,
             "launches_per_step": launches,
-            "avg_launch_us": sweep_s / max(launches, 1) * 1e6,
+            "avg_launch_us": launch_s * 1e6,
+            "achieved_step": value * b_alg / 1e9,
+            "frac_step": value * b_alg / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_dof_update_step": b_alg,
+            "achieved_transport_only": transport_bytes * nelem / sweep_s / 1e9,
             "copy_peak_measured": copy_peak_gbs(),
         },
     }

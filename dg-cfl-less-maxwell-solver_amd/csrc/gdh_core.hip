@@ -418,6 +418,13 @@ extern "C" void gdh_synchronize(gdh_ctx *c)
 	}
 }
 
+extern "C" void gdh_rk3_info(gdh_ctx *c, int out[3])
+{
+	out[0] = c->rk3_tile;
+	out[1] = c->rk3_ntk;
+	out[2] = c->rk3_bs;
+}
+
 extern "C" void gdh_timer_start(gdh_ctx *c)
 {
 	HIPCHECK(hipEventRecord(c->ev0, c->stream));

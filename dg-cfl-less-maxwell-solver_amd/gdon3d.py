@@ -147,6 +147,12 @@ def load_library(path=LIB_PATH):
         "dg_stepper_levels": (C.c_int, [C.c_void_p]),
         "dg_stepper_synchronize": (None, [C.c_void_p]),
         "dg_stepper_error_l2": (C.c_double, [C.c_void_p]),
+        "dg_stepper_energy": (C.c_double, [C.c_void_p]),
+        "gdh_energy": (C.c_double, [C.c_void_p]),
+        "gdh_energy_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+        "gdh_upload_state": (None, [C.c_void_p, c_double_p]),
+        "gdh_download_state": (None, [C.c_void_p, c_double_p]),
         "dg_stepper_finish": (None, [C.c_void_p]),
         "dg_stepper_profile": (None, [C.c_void_p, C.c_int, c_double_p]),
         "dg_stepper_free": (None, [C.c_void_p]),
@@ -176,6 +182,7 @@ def load_library(path=LIB_PATH):
         "gdh_sweep_grid": (C.c_int, [C.c_void_p]),
         "gdh_relax_fused": (C.c_int, [C.c_void_p]),
         "gdh_sweep_info": (None, [C.c_void_p, C.POINTER(C.c_int)]),
+        "gdh_rk3_info": (None, [C.c_void_p, C.POINTER(C.c_int)]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
@@ -352,6 +359,21 @@ class Stepper:
     def error_l2(self):
         return self.lib.dg_stepper_error_l2(self.h)
 
+    def energy(self):
+        """1/2 w^T M w of the current state (binary64 device reduction)."""
+        return self.lib.dg_stepper_energy(self.h)
+
+    def upload(self, wn):
+        """Replace the device state by wn (reference layout of simu->wn)."""
+        wn = np.ascontiguousarray(wn, dtype=np.float64)
+        self.lib.gdh_upload_state(self.lib.dg_stepper_device_context(self.h), dptr(wn))
+
+    def download(self):
+        """The device state in the reference layout (no ghost update)."""
+        out = np.zeros(self.sim.s.wlen)
+        self.lib.gdh_download_state(self.lib.dg_stepper_device_context(self.h), dptr(out))
+        return out
+
     def profile(self, enable):
         out = np.zeros(4)
         self.lib.dg_stepper_profile(self.h, int(enable), dptr(out))
@@ -488,6 +510,20 @@ def sweep_info(stepper):
     lib.gdh_sweep_info(lib.dg_stepper_device_context(stepper.h), out)
     return dict(form=SWEEP_FORMS.get(out[0], str(out[0])), clusters=out[1],
                 cluster_levels=out[2], clustered_items=out[3], grid=out[4])
+
+
+RK3_FORMS = {0: "soa", 1: "tile", 2: "pair"}
+
+
+def rk3_info(stepper):
+    """Form of the RK3 stage: dict(kernel, k2_nontemporal, block)."""
+    if not stepper.h:
+        raise RuntimeError("stepper already freed")
+    lib = load_library()
+    out = (C.c_int * 3)()
+    lib.gdh_rk3_info(lib.dg_stepper_device_context(stepper.h), out)
+    return dict(kernel=RK3_FORMS.get(out[0], str(out[0])), k2_nontemporal=bool(out[1]),
+                block=out[2])
 
 
 def relax_fused(stepper):

@@ -129,6 +129,11 @@ gdn_real dg_stepper_error_l2(gdn_stepper *st)
 	return e;
 }
 
+gdn_real dg_stepper_energy(gdn_stepper *st)
+{
+	return gdh_energy(st->ctx);
+}
+
 void dg_stepper_finish(gdn_stepper *st)
 {
 	if (st->simu->use_kinetic_scheme)

@@ -40,6 +40,8 @@ int dg_stepper_iterations(const gdn_stepper *st);
 void dg_stepper_synchronize(gdn_stepper *st);
 /* L2 error against get_imposed_data at the current time (device reduction) */
 gdn_real dg_stepper_error_l2(gdn_stepper *st);
+/* Discrete energy 1/2 w^T M w of the current state (gdh_energy) */
+gdn_real dg_stepper_energy(gdn_stepper *st);
 void dg_stepper_finish(gdn_stepper *st);
 void dg_stepper_free(gdn_stepper *st);
 

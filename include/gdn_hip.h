@@ -140,6 +140,25 @@ void gdh_apply_relaxation(gdh_ctx *ctx);
  * GDH_IMPOSED_HOST, wexact holds the exact solution at the element nodes. */
 double gdh_error_l2(gdh_ctx *ctx, double t, const double *wexact);
 
+/* Discrete energy 1/2 w^T M w of the device state (M the assembled P2 mass
+ * matrix, src/simulation/simulation.c:218-261; w the macro fields, or
+ * sum_v f_v for the kinetic scheme), element DoFs only, in binary64 with a
+ * fixed reduction order. No counterpart in the reference, which reports no
+ * energy; Maxwell conserves it and the upwind scheme dissipates it. */
+double gdh_energy(gdh_ctx *ctx);
+
+/* The same energy of a state held in device memory by the caller, in the
+ * reference layout of simu->wn (src/simulation/simulation.c:22-40,
+ * 211-214): nv = 0 the macro layout wn[6 ipg + q], nv > 0 the kinetic layout
+ * wn[(q nv + v) neq + ipg], neq = 10 ne + 6 nb; det[ne] the element
+ * determinants (device); the result goes to out[0] (device) on the HIP
+ * stream `stream` (NULL: the null stream), which is synchronised before the
+ * call returns. The partial sums live in a device buffer allocated and freed
+ * by the call. Returns 0, or -1 for invalid arguments. No counterpart in
+ * the reference. */
+int gdh_energy_dev(const double *wn, int nv, int ne, int nb, const double *det, double *out,
+		   void *stream);
+
 /* Event timing of the main kernels: when enabled, each sweep (kinetic) or
  * stage kernel (RK3), relaxation kernel and ghost update is bracketed by HIP
  * events recorded on the context stream (no host wait in between). This call
@@ -148,6 +167,14 @@ double gdh_error_l2(gdh_ctx *ctx, double t, const double *wexact);
  * [1] relaxation, [2] ghost update, [3] sweep or stage launches; the
  * counters are then reset. */
 void gdh_profile(gdh_ctx *ctx, int enable, double *out);
+
+/* Form of the RK3 stage of this context (no counterpart in the reference):
+ * out[0] = 0 one element per thread over the global state, 1 the tile's
+ * state staged in LDS, 2 the same with two lanes per element; out[1] = 1
+ * when k2 is loaded and stored with the non-temporal hint, 0 ordinary
+ * accesses; out[2] = elements per block of the global-state stage. Fixed by
+ * GDN_RK3 / GDN_RK3_K2 or, for batches, by gdh_batch_autotune. */
+void gdh_rk3_info(gdh_ctx *ctx, int out[3]);
 
 /* Wait for all queued work. */
 void gdh_synchronize(gdh_ctx *ctx);

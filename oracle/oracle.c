@@ -1819,6 +1819,39 @@ double orc_error_l2(void *h)
 	return e2;
 }
 
+/* Discrete energy 1/2 w^T M w with M the assembled P2 mass matrix
+ * (src/simulation/simulation.c:218-261: det_e times the reference mass per
+ * component) over the element DoFs; w the macro fields, or sum_v f_v
+ * (src/models/d3q4.c:82-95) for the kinetic scheme. */
+double orc_energy(void *h)
+{
+	tables_init();
+	osim *s = h;
+	const omesh *m = s->m;
+	double acc = 0;
+	for (int e = 0; e < m->nelem; e++) {
+		double cod[3][3], det;
+		elem_geom(m, e, cod, &det);
+		for (int iw = 0; iw < 6; iw++) {
+			double w[NQN];
+			for (int i = 0; i < NQN; i++) {
+				int ipg = NQN * e + i;
+				if (s->scheme) {
+					w[i] = 0;
+					for (int iv = 0; iv < s->nv; iv++)
+						w[i] += s->wn[kidx(s, iw * s->nv + iv, ipg)];
+				} else {
+					w[i] = s->wn[(size_t)6 * ipg + iw];
+				}
+			}
+			for (int i = 0; i < NQN; i++)
+				for (int j = 0; j < NQN; j++)
+					acc += 0.5 * det * w[i] * T_mass[i][j] * w[j];
+		}
+	}
+	return acc;
+}
+
 void orc_sim_free(void *h)
 {
 	osim *s = h;

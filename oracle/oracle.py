@@ -56,6 +56,7 @@ def lib():
             "orc_update_boundary": (None, [C.c_void_p, C.c_double]),
             "orc_init_solution": (None, [C.c_void_p]),
             "orc_error_l2": (C.c_double, [C.c_void_p]),
+            "orc_energy": (C.c_double, [C.c_void_p]),
             "orc_sim_free": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -188,6 +189,10 @@ class OSim:
 
     def error_l2(self):
         return lib().orc_error_l2(self.h)
+
+    def energy(self):
+        """1/2 w^T M w of the current state (assembled P2 mass)."""
+        return lib().orc_energy(self.h)
 
     def free(self):
         if self.h:

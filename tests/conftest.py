@@ -16,13 +16,13 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build the library and the oracle only when nothing is built yet."""
-    lib = os.path.join(PKG, "lib", "libgdon3d_hip.so")
-    orc = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not os.path.exists(lib):
-        subprocess.run(["make", "-C", PKG, "-j8"], check=True)
-    if not os.path.exists(orc):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    """Rebuild the library and the oracle whenever make finds them older
+    than their sources (make -q: nothing runs when they are up to date), so
+    that a stale library is never what gets tested."""
+    for d, extra in ((PKG, ["-j8"]), (os.path.join(ROOT, "oracle"), [])):
+        if subprocess.run(["make", "-q", "-C", d], stdout=subprocess.DEVNULL,
+                          stderr=subprocess.DEVNULL).returncode != 0:
+            subprocess.run(["make", "-C", d] + extra, check=True)
 
 
 _ensure_built()

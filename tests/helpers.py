@@ -65,3 +65,41 @@ def twisted_msh(n, alpha):
         with open(path, "w") as f:
             f.write("\n".join(lines))
     return path
+
+
+class HipRuntime:
+    """The HIP runtime the library itself loaded (same soname, so the same
+    handle), for tests that hand device pointers to the C ABI: hipMalloc,
+    hipMemcpy, hipFree."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        G.load_library()
+        self.h = C.CDLL("libamdhip64.so.7")
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipDeviceSynchronize.argtypes = []
+
+    def check(self, rc, what):
+        assert rc == 0, f"{what} returned {rc}"
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.h.hipMalloc(C.byref(p), max(int(nbytes), 16)), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self.check(self.h.hipFree(p), "hipFree")
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        self.check(self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy")
+        return p
+
+    def to_host(self, p, n, dtype=np.float64):
+        out = np.zeros(n, dtype)
+        self.check(self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, self.D2H), "hipMemcpy")
+        return out

@@ -222,3 +222,32 @@ def test_time_parameters_of_the_benchmark_configurations():
     assert sim.s.itermax == 321 and sim.s.neq == 2039808 and sim.s.m == 24
     assert abs(sim.s.dt - 6.228e-3) < 1e-6
     sim.free()
+
+
+@pytest.mark.parametrize("scheme,jitter", [(0, 0.0), (0, 0.25), (1, 0.1)])
+def test_energy_exact_for_quadratic_fields(scheme, jitter):
+    """The oracle's energy 1/2 w^T M w against closed forms: for fields that
+    are at most quadratic the P2 interpolant is the field itself and M is the
+    exact mass, so the energy is 1/2 of the exact integral over the unit
+    cube: E_x = 1 gives 1/2, E_y = x gives 1/6, H_z = x y gives 1/18,
+    E_z = x^2 gives 1/10 (O(1) off for a lumped mass, a missing 1/2 or a
+    wrong det). Kinetic states carry w / nv in every velocity."""
+    path = kuhn_msh(3, jitter, 5)
+    om, X = dof_coords(path)
+    kin = O.D3Q4 if scheme else 0
+    s = O.OSim(om, scheme, O.PLANE_WAVE, plane_wave_par(), kin=kin)
+    x, y = X[:, 0], X[:, 1]
+    fields = [(0, np.ones_like(x), 0.5), (1, x, 1 / 6), (5, x * y, 1 / 18), (2, x * x, 0.1)]
+    for q, val, exact in fields:
+        w = np.zeros((len(x), 6))
+        w[:, q] = val
+        if scheme:
+            nv = s.m // 6
+            f = np.repeat(w.T[:, None, :], nv, axis=1) / nv  # [q][v][ipg]
+            s.set_wn(f.reshape(-1))
+        else:
+            s.set_wn(w.reshape(-1))
+        e = s.energy()
+        assert abs(e - exact) <= 1e-13, (q, e, exact)
+    s.free()
+    om.free()

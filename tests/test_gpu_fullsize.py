@@ -24,6 +24,7 @@ def cavity_32(steps, sweep, precision="fp64"):
     sim.set_time(2.0, 0.0, 5.0)
     st = G.Stepper(sim, precision=precision)
     info = G.sweep_info(st)
+    fused = G.relax_fused(st)
     st.advance(steps)
     st.sync()
     t, err = st.t, st.error_l2()
@@ -32,25 +33,33 @@ def cavity_32(steps, sweep, precision="fp64"):
     dt = sim.s.dt
     st.free()
     sim.free()
-    return dict(wn=wn, t=t, err=err, dt=dt, info=info)
+    return dict(wn=wn, t=t, err=err, dt=dt, info=info, fused=fused)
 
 
 @pytest.mark.gpu
 def test_cavity_32cubed_sweep_forms_agree(monkeypatch):
-    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    """The benchmarked form (item sweep, relaxation fused into it), the item
+    sweep with the relaxation as its own pass and the cluster sweep, each
+    forced and checked, on the full 32^3 cavity."""
     out = {}
-    for form in ("item", "cluster"):
+    for form, fused in (("item", "0"), ("cluster", "0"), ("item", "1")):
         monkeypatch.setenv("GDN_SWEEP", form)
-        out[form] = cavity_32(30, form)
-        assert out[form]["info"]["form"] == form
-        print(f"32^3 cavity, {form} sweep: t={out[form]['t']:.6f} L2 error {out[form]['err']:.4e}",
+        monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+        key = f"{form}-fused{fused}"
+        out[key] = cavity_32(30, form)
+        assert out[key]["info"]["form"] == form
+        assert out[key]["fused"] == int(fused)
+        print(f"32^3 cavity, {key}: t={out[key]['t']:.6f} L2 error {out[key]['err']:.4e}",
               flush=True)
-    a, b = out["item"], out["cluster"]
-    assert a["t"] == b["t"] and abs(a["t"] - 30 * a["dt"]) <= 1e-12
-    r = rel_l2(b["wn"], a["wn"])
-    print(f"  item vs cluster: relative L2 {r:.3e}", flush=True)
-    assert r <= 1e-13
-    assert a["err"] < 1e-3 and abs(a["err"] - b["err"]) <= 1e-12 * a["err"]
+    a = out["item-fused0"]
+    assert abs(a["t"] - 30 * a["dt"]) <= 1e-12 and a["err"] < 1e-3
+    for key in ("cluster-fused0", "item-fused1"):
+        b = out[key]
+        r = rel_l2(b["wn"], a["wn"])
+        print(f"  item-fused0 vs {key}: relative L2 {r:.3e}", flush=True)
+        assert a["t"] == b["t"]
+        assert r <= 1e-13
+        assert abs(a["err"] - b["err"]) <= 1e-12 * a["err"]
 
 
 def plane_64(steps, precision):

@@ -1,6 +1,11 @@
 """GPU parity of the CFL-less relaxation solve (dg_solve_relaxation through
-the C ABI: level-scheduled block sweep + node relaxation) against the
-oracle's assembled theta-scheme with sparse LU (src/dg/relaxation.c:24-185).
+the C ABI: upwind block sweep + node relaxation) against the oracle's
+assembled theta-scheme with sparse LU (src/dg/relaxation.c:24-185).
+
+Every oracle comparison names the form of the transport sweep it checks
+(GDN_SWEEP = level | item | cluster, GDN_RELAX_FUSED = 0 | 1) and asserts
+that the library ran that form; the benchmarked form is the item sweep with
+the relaxation fused into it (item, 1).
 
 Tolerance: relative L2 <= 1e-12 on the full micro state wn (FP64)."""
 import os
@@ -13,9 +18,55 @@ import oracle as O
 from helpers import kuhn_msh, plane_wave_par, cavity_par, rel_l2
 
 TOL = 1e-12
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# (GDN_SWEEP, GDN_RELAX_FUSED): every form the library can run; the cluster
+# and level sweeps take a relaxed state (relaxation as a pass of its own)
+FORMS = [("item", "1"), ("item", "0"), ("cluster", "0"), ("level", "0")]
+FORM_IDS = [f"{s}-fused{f}" for s, f in FORMS]
+
+_ORACLE_CACHE = {}
+
+
+def oracle_run(path, kind, par, kin, tend, cfl):
+    """The oracle's solve, once per case and process."""
+    key = (path, kind, tuple(np.asarray(par).ravel()), kin, tend, cfl)
+    if key not in _ORACLE_CACHE:
+        om = O.OMesh(path)
+        s = O.OSim(om, 1, kind, par, kin=O.D3Q4 if kin == "d3q4" else O.D3Q6)
+        s.set_time(tend, 0.0, cfl)
+        s.solve()
+        _ORACLE_CACHE[key] = dict(wn=s.wn(), err=s.error_l2(), itermax=s.itermax, t=s.t)
+        s.free()
+        om.free()
+    return _ORACLE_CACHE[key]
+
+
+def gpu_run(path, imposed, kin, tend, cfl, form, monkeypatch):
+    """dg_solve_relaxation's sequence (create, init, advance itermax steps,
+    final ghost update and download) with the sweep form forced, and the
+    form the library actually ran."""
+    sweep, fused = form
+    monkeypatch.setenv("GDN_SWEEP", sweep)
+    monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+    mesh = G.Mesh(path=path)
+    sim = G.Simulation(mesh, G.Model(imposed, kinetic=kin))
+    sim.set_time(tend, 0.0, cfl)
+    st = G.Stepper(sim)
+    ran = (G.sweep_info(st)["form"], str(G.relax_fused(st)))
+    st.advance(sim.s.itermax)
+    st.sync()
+    err = st.error_l2()
+    st.finish()
+    g = dict(wn=sim.wn(), err=err, itermax=sim.s.itermax, t=st.t, form=ran)
+    st.free()
+    sim.free()
+    return g
 
 
 def run_pair(path, imposed, kind, par, kin, tend, cfl):
+    """dg_solve itself (the drop-in entry point, form chosen by the
+    library) and the oracle."""
     mesh = G.Mesh(path=path)
     model = G.Model(imposed, kinetic=kin)
     sim = G.Simulation(mesh, model)
@@ -23,21 +74,51 @@ def run_pair(path, imposed, kind, par, kin, tend, cfl):
     sim.solve()
     g = dict(wn=sim.wn(), err=sim.error_l2(), itermax=sim.s.itermax, t=sim.s.t)
     sim.free()
-    om = O.OMesh(path)
-    s = O.OSim(om, 1, kind, par, kin=O.D3Q4 if kin == "d3q4" else O.D3Q6)
-    s.set_time(tend, 0.0, cfl)
-    s.solve()
-    o = dict(wn=s.wn(), err=s.error_l2(), itermax=s.itermax, t=s.t)
-    s.free()
-    om.free()
-    return g, o
+    return g, oracle_run(path, kind, par, kin, tend, cfl)
+
+
+def check_form(path, imposed, kind, par, kin, tend, cfl, form, monkeypatch):
+    g = gpu_run(path, imposed, kin, tend, cfl, form, monkeypatch)
+    o = oracle_run(path, kind, par, kin, tend, cfl)
+    r = rel_l2(g["wn"], o["wn"])
+    print(f"{os.path.basename(path)} {kin} form {form}: ran {g['form']}, "
+          f"relative L2 vs oracle {r:.3e}", flush=True)
+    assert g["form"] == form
+    assert g["itermax"] == o["itermax"]
+    assert r <= TOL
+    assert abs(g["err"] - o["err"]) <= 1e-10 * o["err"]
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("n,jitter,tend,cfl", [(3, 0.0, 0.3, 5.0), (4, 0.15, 0.3, 5.0),
                                                (6, 0.0, 0.5, 5.0)])
-def test_relax_cavity_d3q4(n, jitter, tend, cfl):
-    print(f"relax cavity n={n} jitter={jitter}", flush=True)
+def test_relax_cavity_d3q4_forms(n, jitter, tend, cfl, form, monkeypatch):
+    """Each sweep form against the oracle on the cavity mode: the 3^3 and
+    6^3 Kuhn cubes and the 4^3 cube with +-15 % jitter."""
+    path = kuhn_msh(n, jitter, 23)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    check_form(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", tend, cfl, form,
+               monkeypatch)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+@pytest.mark.parametrize("kin", ["d3q4", "d3q6"])
+def test_relax_reference_mesh_forms(kin, form, monkeypatch):
+    """Each sweep form against the oracle on the reference's own
+    unstructured mesh data/mesh/t4_cube_8.msh (1808 tets)."""
+    path = os.path.join(GOLDEN, "t4_cube_8.msh")
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    check_form(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), kin, 0.2, 5.0, form,
+               monkeypatch)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,jitter,tend,cfl", [(3, 0.0, 0.3, 5.0), (4, 0.15, 0.3, 5.0)])
+def test_dg_solve_relaxation_cavity(n, jitter, tend, cfl):
+    """dg_solve (the drop-in entry point, form chosen by timing) against
+    the oracle."""
     path = kuhn_msh(n, jitter, 23)
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
     g, o = run_pair(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", tend, cfl)
@@ -47,21 +128,22 @@ def test_relax_cavity_d3q4(n, jitter, tend, cfl):
 
 
 @pytest.mark.gpu
-def test_relax_plane_wave_d3q4_jittered():
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_relax_plane_wave_d3q4_jittered(form, monkeypatch):
     path = kuhn_msh(4, 0.2, 7)
     k = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
     G.set_plane_wave(k, (0, 0, 1))
-    g, o = run_pair(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(k, (0, 0, 1)),
-                    "d3q4", 0.3, 2.0)
-    assert rel_l2(g["wn"], o["wn"]) <= TOL
+    check_form(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(k, (0, 0, 1)), "d3q4", 0.3,
+               2.0, form, monkeypatch)
 
 
 @pytest.mark.gpu
-def test_relax_d3q6():
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_relax_d3q6(form, monkeypatch):
     path = kuhn_msh(3, 0.1, 9)
     G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
-    g, o = run_pair(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(), "d3q6", 0.25, 3.0)
-    assert rel_l2(g["wn"], o["wn"]) <= TOL
+    check_form(path, "plane_wave", O.PLANE_WAVE, plane_wave_par(), "d3q6", 0.25, 3.0, form,
+               monkeypatch)
 
 
 @pytest.mark.gpu
@@ -101,6 +183,7 @@ def test_fused_relaxation_equals_separate(kin, precision, monkeypatch):
     second advance."""
     path = kuhn_msh(6, 0.1, 41)
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    monkeypatch.setenv("GDN_SWEEP", "item")
     out = {}
     for fused in ("0", "1"):
         monkeypatch.setenv("GDN_RELAX_FUSED", fused)
@@ -109,6 +192,7 @@ def test_fused_relaxation_equals_separate(kin, precision, monkeypatch):
         sim.set_time(0.4, 0.0, 5.0)
         st = G.Stepper(sim, precision=precision)
         assert G.relax_fused(st) == int(fused)
+        assert G.sweep_info(st)["form"] == "item"
         half = sim.s.itermax // 2
         st.advance(half)
         err_mid = st.error_l2()
@@ -187,9 +271,6 @@ def test_cluster_sweep_parity_with_oracle(kin, n, jitter, monkeypatch):
     assert r <= TOL
 
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("mesh,kin,imposed", [("t4_cube_8", "d3q4", "cavity"),
                                               ("t4_cube_8", "d3q6", "cavity"),
@@ -217,15 +298,12 @@ def test_cluster_sweep_unstructured_meshes(mesh, kin, imposed, monkeypatch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n", [1, 2])
-@pytest.mark.parametrize("form", ["level", "item", "cluster"])
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
 def test_smallest_meshes_every_sweep_form(n, form, monkeypatch):
     """Edge sizes: one Kuhn cube (6 tets, one cluster per velocity, every
     element next to the boundary) and 2^3 cubes, every form of the sweep
     against the oracle: relative L2 <= 1e-12."""
-    monkeypatch.setenv("GDN_SWEEP", form)
-    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
     path = kuhn_msh(n, 0.0, 0)
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
-    g, o = run_pair(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", 0.5, 5.0)
-    assert g["itermax"] == o["itermax"]
-    assert rel_l2(g["wn"], o["wn"]) <= TOL
+    check_form(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", 0.5, 5.0, form,
+               monkeypatch)

@@ -147,7 +147,9 @@ def test_rk3_k2_stream_hints_bit_identical(precision, monkeypatch):
     (GDN_RK3_K2=nt, the default for one mesh) or ordinary accesses
     (GDN_RK3_K2=plain, which the batch autotuner may pick): a cache hint
     changes no value, so every stage kernel gives the same state bit for bit
-    after a whole solve (6^3 x 6 tets, jittered, upwind boundary)."""
+    after a whole solve (6^3 x 6 tets, jittered, upwind boundary). The
+    context reports the kernel and hint it runs, so a library that ignored
+    the variables would fail here."""
     path = kuhn_msh(6, 0.1, 3)
     G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
     for kern in ("soa", "tile", "pair"):
@@ -159,6 +161,9 @@ def test_rk3_k2_stream_hints_bit_identical(precision, monkeypatch):
             sim = G.Simulation(mesh, G.Model("plane_wave"))
             sim.set_time(0.1, 0.0, 0.5)
             st = G.Stepper(sim, precision=precision)
+            info = G.rk3_info(st)
+            assert info["kernel"] == kern
+            assert info["k2_nontemporal"] == (hint == "nt")
             st.advance(sim.s.itermax)
             st.sync()
             st.finish()
