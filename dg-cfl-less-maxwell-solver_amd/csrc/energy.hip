@@ -10,6 +10,8 @@
  * Two passes with a fixed reduction order (per-block partial sums, then one
  * 64-lane block), so a result does not depend on scheduling.
  */
+#include <pthread.h>
+
 #include "ctx.h"
 
 #define EN_BS 256
@@ -115,15 +117,16 @@ static int energy_blocks(long ne)
 	return (int)std::min<long>(EN_MAXB, std::max<long>(1, (ne + EN_BS - 1) / EN_BS));
 }
 
-/* both passes on stream s into out (device), partial sums in scratch */
+/* both passes on stream s into out (device), the partial sums (at most
+ * EN_MAXB doubles) in part */
 template <class A>
-static void energy_launch(const A &a, const double *det, int dstride, long ne, double *scratch,
+static void energy_launch(const A &a, const double *det, int dstride, long ne, double *part,
 			  double *out, hipStream_t s)
 {
 	const int np = energy_blocks(ne);
-	k_energy_partial<A><<<np, EN_BS, 0, s>>>(a, det, dstride, ne, scratch);
+	k_energy_partial<A><<<np, EN_BS, 0, s>>>(a, det, dstride, ne, part);
 	HIPCHECK(hipGetLastError());
-	k_energy_final<<<1, 64, 0, s>>>(scratch, np, out);
+	k_energy_final<<<1, 64, 0, s>>>(part, np, out);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -182,9 +185,65 @@ extern "C" int gdh_energy_dev(const double *wn, int nv, int ne, int nb, const do
 		return -1;
 	const hipStream_t s = (hipStream_t)stream;
 	const EnergyRef a{ wn, nv, 10L * ne + 6L * nb };
-	double *scratch = (double *)dmalloc(sizeof(double) * EN_MAXB);
-	energy_launch(a, det, 1, ne, scratch, out, s);
+	double *part = (double *)dmalloc(sizeof(double) * EN_MAXB);
+	energy_launch(a, det, 1, ne, part, out, s);
 	HIPCHECK(hipStreamSynchronize(s));
-	HIPCHECK(hipFree(scratch));
+	HIPCHECK(hipFree(part));
+	return 0;
+}
+
+/*
+ * The energy without any allocation: the partial sums go to a fixed device
+ * array of EN_MAXB doubles (8 KiB, held on every GPU the library runs on
+ * until the process ends) instead of a buffer allocated per call. Its device
+ * address is looked up once per device, with that device current, at the
+ * first call on it; host code uses that address, never the array's name.
+ * Every partial sum is written before it is read, so nothing needs clearing;
+ * calls that overlap on one GPU share the array and race.
+ */
+static __device__ __align__(256) double scratch[EN_MAXB];
+
+#define EN_MAX_DEVICES 64
+static double *scratch_dev[EN_MAX_DEVICES];
+static pthread_mutex_t scratch_lock = PTHREAD_MUTEX_INITIALIZER;
+
+/* the address of this device's copy of scratch, or null on failure */
+static double *scratch_of_current_device()
+{
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= EN_MAX_DEVICES) {
+		fprintf(stderr, "gdon3d-hip: gdh_energy_dev_noalloc: no usable current device\n");
+		return nullptr;
+	}
+	pthread_mutex_lock(&scratch_lock);
+	if (!scratch_dev[dev]) {
+		void *p = nullptr;
+		const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (rc != hipSuccess)
+			fprintf(stderr, "gdon3d-hip: hipGetSymbolAddress(scratch) failed: %s\n",
+				hipGetErrorString(rc));
+		else
+			scratch_dev[dev] = (double *)p;
+	}
+	double *p = scratch_dev[dev];
+	pthread_mutex_unlock(&scratch_lock);
+	return p;
+}
+
+extern "C" int gdh_energy_dev_noalloc(const double *wn, int nv, int ne, int nb,
+				      const double *det, double *out, void *stream)
+{
+	if (!energy_args_ok(wn, nv, ne, nb, det, out))
+		return -1;
+	if (energy_blocks(ne) > EN_MAXB) { /* the grid is capped: never taken */
+		fprintf(stderr, "gdon3d-hip: gdh_energy_dev_noalloc: %d partial sums exceed the "
+				"fixed scratch of %d\n", energy_blocks(ne), EN_MAXB);
+		return -1;
+	}
+	double *part = scratch_of_current_device();
+	if (!part)
+		return -1;
+	const EnergyRef a{ wn, nv, 10L * ne + 6L * nb };
+	energy_launch(a, det, 1, ne, part, out, (hipStream_t)stream);
 	return 0;
 }

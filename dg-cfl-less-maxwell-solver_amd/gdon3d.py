@@ -151,6 +151,8 @@ def load_library(path=LIB_PATH):
         "gdh_energy": (C.c_double, [C.c_void_p]),
         "gdh_energy_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
+        "gdh_energy_dev_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
         "gdh_upload_state": (None, [C.c_void_p, c_double_p]),
         "gdh_download_state": (None, [C.c_void_p, c_double_p]),
         "dg_stepper_finish": (None, [C.c_void_p]),

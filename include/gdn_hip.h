@@ -159,6 +159,19 @@ double gdh_energy(gdh_ctx *ctx);
 int gdh_energy_dev(const double *wn, int nv, int ne, int nb, const double *det, double *out,
 		   void *stream);
 
+/* gdh_energy_dev without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. Same arguments
+ * and result (bit for bit: the same kernels and reduction order); the
+ * partial sums go to a fixed device array of 1024 doubles (8 KiB per GPU,
+ * held until the process ends) instead of a per-call buffer, so every input
+ * size fits it. The call only enqueues the two kernels on `stream` and does
+ * not synchronise: out[0] is valid once the stream has reached them. Calls
+ * that overlap on one GPU share the array and race; serialise them (one
+ * stream). Returns 0, or -1 for invalid arguments (nothing enqueued). */
+int gdh_energy_dev_noalloc(const double *wn, int nv, int ne, int nb, const double *det,
+			   double *out, void *stream);
+
 /* Event timing of the main kernels: when enabled, each sweep (kinetic) or
  * stage kernel (RK3), relaxation kernel and ghost update is bracketed by HIP
  * events recorded on the context stream (no host wait in between). This call
