@@ -107,6 +107,40 @@ void d3q4_get_feq(const void *self, const gdn_real *w, gdn_real *feq)
 	d3q4_apply_M_inv(m, w, qM, feq);
 }
 
+/* (w, y) = Q(f): the moments w = sum_v f, q = sum_v v f and their distance
+ * y = q - qM(w) to the physical flux (src/models/d3q4.c:200-225) */
+void d3q4_apply_Q(const void *self, const gdn_real *f, gdn_real *w, gdn_real *y)
+{
+	const gdn_model *m = self;
+	gdn_real q[3 * 64], qM[3 * 64];
+	d3q4_apply_M(m, f, w, q);
+	d3q4_get_qM(m, w, qM);
+	for (int k = 0; k < 3 * m->nb_w; k++)
+		y[k] = q[k] - qM[k];
+}
+
+/* f = Q^-1(w, y) = M^-1(w, y + qM(w)) (src/models/d3q4.c:227-252) */
+void d3q4_apply_Q_inv(const void *self, const gdn_real *w, const gdn_real *y, gdn_real *f)
+{
+	const gdn_model *m = self;
+	gdn_real q[3 * 64];
+	d3q4_get_qM(m, w, q);
+	for (int k = 0; k < 3 * m->nb_w; k++)
+		q[k] = y[k] + q[k];
+	d3q4_apply_M_inv(m, w, q, f);
+}
+
+/* src/models/d3q4.c:334-352: f = Q^-1(w_inc, y) with every component of
+ * y_iw equal to w_inc[iw] */
+void test_philippe_d3q4_get_f(const void *self, const gdn_real *w_inc, gdn_real *f)
+{
+	const gdn_model *m = self;
+	gdn_real y[3 * 64];
+	for (int iw = 0; iw < m->nb_w; iw++)
+		y[3 * iw + 0] = y[3 * iw + 1] = y[3 * iw + 2] = w_inc[iw];
+	d3q4_apply_Q_inv(m, w_inc, y, f);
+}
+
 /* The reference evaluates the boundary state as f_eq of the imposed data
  * (use_feq = true, src/models/d3q4.c:295-301); fL and vn are unused. */
 void d3q4_get_fR(const void *self, const gdn_real *fL, const gdn_real *vn,
@@ -115,4 +149,19 @@ void d3q4_get_fR(const void *self, const gdn_real *fL, const gdn_real *vn,
 	(void)fL;
 	(void)vn;
 	d3q4_get_feq(self, wI, fR);
+}
+
+/* The reference's other boundary state, the branch its use_feq = false
+ * selects (src/models/d3q4.c:303-332): (wL, yL) = Q(fL), wR = P^-(vn) wI
+ * through the model's get_proj_neg (the reference overwrites the sum
+ * P^{+,0} wL + P^- wI with P^- wI), fR = Q^-1(wR, yL). A model selects it by
+ * pointing get_fR here; the GPU solver runs the same on the device. */
+void d3q4_get_fR_projector(const void *self, const gdn_real *fL, const gdn_real *vn,
+			   const gdn_real *wI, gdn_real *fR)
+{
+	const gdn_model *m = self;
+	gdn_real wL[64], wR[64], yL[3 * 64];
+	d3q4_apply_Q(m, fL, wL, yL);
+	m->get_proj_neg(wI, vn, wR);
+	d3q4_apply_Q_inv(m, wR, yL, fR);
 }

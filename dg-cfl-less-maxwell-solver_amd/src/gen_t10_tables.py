@@ -23,6 +23,11 @@ Tables (all exact except ``FACE_SCAL``):
   inverse mass applied to the volume operator of direction k;
 * ``MINV_LIFT[f][m][b] = sum_a INV_MASS[m][node(f, a)] FACE_SCAL[a][b]``:
   the inverse mass applied to the face-f lift of the (rounded) face mass;
+* ``FLUX_REF[i][j][f] = int_{face f} phi_i phi_j dS``, the exact face mass of
+  the reference tetrahedron (faces 0..2 of area 1/2, face 3 of area sqrt(3)/2,
+  so its entries carry a factor sqrt(3), evaluated to 60 digits and rounded
+  once); the reference tabulates it (src/mesh/t10.c:693-1315) but its
+  operators use FACE_SCAL;
 * ``FACE_SCAL[a][b]``: the P2 mass matrix of the area-1/2 reference triangle
   in face-node order (v0, v1, v2, e01, e12, e20), rounded to 10 significant
   decimal digits.  The reference stores exactly that rounding
@@ -183,6 +188,24 @@ def main(out=sys.stdout):
     w("/* int d_k(phi_i) phi_j, indexed [i][j][k] */\n#define T10_INTERNAL_INIT { \\\n")
     for i in range(10):
         w("\t{ " + ", ".join("{ " + ", ".join(lit(internal[i][j][k]) for k in range(3)) + " }"
+                             for j in range(10)) + " }, \\\n")
+    w("}\n\n")
+    getcontext().prec = 60
+    sqrt3 = Decimal(3).sqrt()
+    tri_exact = [[integrate_tri(pmul(p, q)) for q in TRI] for p in TRI]
+    f2n_ = [(0, 3, 2, 7, 9, 6), (0, 1, 3, 4, 8, 7), (0, 2, 1, 6, 5, 4), (1, 2, 3, 5, 9, 8)]
+    flux = [[[Fr(0)] * 4 for _ in range(10)] for _ in range(10)]
+    for f in range(4):
+        for a in range(6):
+            for b in range(6):
+                v = tri_exact[a][b]
+                if f == 3:
+                    v = Fr(Decimal(v.numerator) / Decimal(v.denominator) * sqrt3)
+                flux[f2n_[f][a]][f2n_[f][b]][f] = v
+    w("/* int over face f of phi_i phi_j (exact face mass; face 3 of area sqrt(3)/2),\n"
+      " * indexed [i][j][f] */\n#define T10_FLUX_REF_INIT { \\\n")
+    for i in range(10):
+        w("\t{ " + ", ".join("{ " + ", ".join(lit(flux[i][j][f]) for f in range(4)) + " }"
                              for j in range(10)) + " }, \\\n")
     w("}\n\n")
     emit2("T10_FACE_SCAL_INIT", face,

@@ -104,3 +104,49 @@ void gdn_tetra_ref2phy(gdn_real physnode[4][3], gdn_real xref[3],
 		}
 	}
 }
+
+/*
+ * Inverse of the affine map (src/mesh/tetrageometry.c:262-292). The
+ * reference runs 20 Newton steps from xref = (1/2, 1/2, 1/2); the map being
+ * affine, the first step already lands on the solution and the others only
+ * re-round it. Here: the step from (1/2, 1/2, 1/2), xref = c - codtau^T
+ * (x(c) - xphy) / det, then one correction step from that point.
+ */
+void gdn_tetra_phy2ref(gdn_real physnode[4][3], gdn_real xphy[3], gdn_real xref[3])
+{
+	gdn_real dtau[3][3], codtau[3][3], x[3];
+	xref[0] = xref[1] = xref[2] = 0.5;
+	for (int step = 0; step < 2; step++) {
+		gdn_tetra_ref2phy(physnode, xref, NULL, -1, x, dtau, codtau, NULL, NULL);
+		const gdn_real det = geometry_dot_product(dtau[0], codtau[0]);
+		const gdn_real r[3] = { x[0] - xphy[0], x[1] - xphy[1], x[2] - xphy[2] };
+		for (int i = 0; i < 3; i++)
+			xref[i] -= (codtau[0][i] * r[0] + codtau[1][i] * r[1] +
+				    codtau[2][i] * r[2]) / det;
+	}
+}
+
+/*
+ * src/mesh/tetrageometry.c:294-348: the reference follows the homotopy
+ * xphy(theta) = theta xphy + (1 - theta) x(1/2, 1/2, 1/2), theta = k / 100,
+ * and returns at the first theta whose reference point leaves
+ * [-eps, 1 + eps]^3 (eps = DBL_EPSILON). Affine maps take that segment to the
+ * segment xref(theta) = theta xref + (1 - theta) (1/2, 1/2, 1/2), so the
+ * inverse is formed once and the path walked on the reference side.
+ */
+void gdn_tetra_robust_phy2ref(gdn_real physnode[4][3], gdn_real xphy[3], gdn_real xref[3])
+{
+	gdn_real xr[3];
+	gdn_tetra_phy2ref(physnode, xphy, xr);
+	const gdn_real dtheta = 1. / 100;
+	for (int k = 0; k <= 100; k++) {
+		const gdn_real theta = k * dtheta;
+		int inside = 1;
+		for (int i = 0; i < 3; i++) {
+			xref[i] = theta * xr[i] + (1 - theta) * 0.5;
+			inside &= xref[i] >= _GDN_XREF_LOWBOUND && xref[i] <= _GDN_XREF_UPBOUND;
+		}
+		if (!inside)
+			return;
+	}
+}

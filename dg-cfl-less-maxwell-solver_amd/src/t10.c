@@ -34,6 +34,7 @@ static const gdn_real mass[NQN][NQN] = T10_MASS_INIT;
 static const gdn_real inv_mass[NQN][NQN] = T10_INV_MASS_INIT;
 static const gdn_real internal[NQN][NQN][3] = T10_INTERNAL_INIT;
 static const gdn_real face_scal[NQV][NQV] = T10_FACE_SCAL_INIT;
+static const gdn_real flux_ref[NQN][NQN][NFA] = T10_FLUX_REF_INIT;
 
 /* P1 element: faces as vertex triples and the nodal weights 1/24. */
 static const int t4_face2node[NFA][3] = { { 0, 3, 2 },
@@ -120,6 +121,12 @@ gdn_real t10_get_inv_mass_ref_entry(const int i, const int j)
 gdn_real t10_get_internal_ref_entry(const int i, const int j, const int id_dir)
 {
 	return internal[i][j][id_dir];
+}
+
+/* src/mesh/t10.c:1428-1431: the exact face mass of face id_face */
+gdn_real t10_get_flux_ref_entry(const int i, const int j, const int id_face)
+{
+	return flux_ref[i][j][id_face];
 }
 
 gdn_real t10_get_face_scale_entry(const int i, const int j)

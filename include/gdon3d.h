@@ -6,7 +6,8 @@
  * build always defines it (CMakeLists.txt:26); this library is built for
  * double only, so gdn_real is double unconditionally and every struct of the
  * API keeps the reference's double-precision layout. Reduced-precision field
- * storage is a run-time option of the GPU solver (gdn_hip_set_precision),
+ * storage is a run-time option of the GPU solver
+ * (dg_stepper_create_precision),
  * not a change of gdn_real.
  */
 #ifndef GDON3D_H

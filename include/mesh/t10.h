@@ -41,6 +41,8 @@ gdn_real t10_get_wpg_ref_entry(const int i);
 gdn_real t10_get_mass_ref_entry(const int i, const int j);
 gdn_real t10_get_inv_mass_ref_entry(const int i, const int j);
 gdn_real t10_get_internal_ref_entry(const int i, const int j, const int id_dir);
+/* int over reference face id_face of phi_i phi_j (src/mesh/t10.h:85) */
+gdn_real t10_get_flux_ref_entry(const int i, const int j, const int id_face);
 gdn_real t10_get_face_scale_entry(const int i, const int j);
 
 /* P1 reference tetrahedron (src/mesh/t4.h): face -> vertex map and weights */
