@@ -58,6 +58,7 @@ struct gdh_ctx {
 
 	double *geom;
 	int4 *nbr, *nbmap;
+	int2 *bface; /* kinetic: per boundary face, its element and local face */
 	double *xnode, *xghost;
 	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
 	 * GH[q][6 bf + a]; kinetic: w = item-major state (kin_at), ghosts

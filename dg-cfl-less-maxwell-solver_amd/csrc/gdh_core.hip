@@ -121,6 +121,9 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	memcpy(c->kp.vi, model->vi, sizeof(c->kp.vi));
 	c->kp.lambda = model->lambda;
 	c->kp.omega = model->omega;
+	c->kp.fr_kind = model->fr_kind;
+	memcpy(c->kp.proj_pos, model->proj_pos, sizeof(c->kp.proj_pos));
+	memcpy(c->kp.proj_neg, model->proj_neg, sizeof(c->kp.proj_neg));
 	if (scheme && c->kp.kind == GDH_KIN_NONE) {
 		fprintf(stderr, "gdon3d-hip: kinetic scheme without velocity set\n");
 		abort();
@@ -159,11 +162,24 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->geom_soa = nullptr;
 	c->elem_pos = nullptr;
 	c->stage = nullptr;
+	c->bface = nullptr;
 	if (scheme) {
 		/* the sweep reads neighbours in mesh order */
 		HIPCHECK(hipMemcpy(c->nbr, mesh->nbr, sizeof(int4) * ne,
 				   hipMemcpyHostToDevice));
 		HIPCHECK(hipMemcpy(c->nbmap, mesh->nbmap, sizeof(int4) * ne,
+				   hipMemcpyHostToDevice));
+		/* element and local face of each boundary face (projector
+		 * boundary state) */
+		std::vector<int2> bf(nb ? nb : 1, make_int2(0, 0));
+		for (size_t e = 0; e < ne; e++)
+			for (int f = 0; f < 4; f++) {
+				const int x = mesh->nbr[4 * e + f];
+				if (x < 0 && -1 - x < (int)nb)
+					bf[-1 - x] = make_int2((int)e, f);
+			}
+		c->bface = (int2 *)dmalloc(sizeof(int2) * bf.size());
+		HIPCHECK(hipMemcpy(c->bface, bf.data(), sizeof(int2) * bf.size(),
 				   hipMemcpyHostToDevice));
 	} else {
 		/* RK3 works on storage columns: geometry, neighbours (as columns)
@@ -287,7 +303,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev, c->chead, c->cdesc,
 			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
 			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
-			 c->stage };
+			 c->stage, c->bface };
 	for (void *p : ptrs)
 		if (p)
 			HIPCHECK(hipFree(p));
@@ -563,6 +579,63 @@ extern "C" void gdh_init_solution(gdh_ctx *c, const double *wexact)
 	fill_launch(c, 0, n, c->xnode, hv, 0.0, 0.0);
 }
 
+/*
+ * Ghost DoFs of the projector boundary state (the reference's get_fR with
+ * use_feq = false, src/dg/dg.c:75-100): ghost k = 6 bf + a takes
+ * fR = fr_projector(fL, vn, wI) with fL the state at the L element's node
+ * of face node a, vn the outward normal of that face and wI the imposed
+ * data at the node (closed form, or hv[6 k] from the host). The projectors
+ * depend on vn / |vn| only, so the operator's normal codtau n_ref serves.
+ */
+template <typename T, int NV>
+__global__ void k_ghost_projector(T *__restrict__ kin, KinPar kp, ImposedPar ip, long ne,
+				  long neq, long n, const int2 *__restrict__ bface,
+				  const double *__restrict__ geom, const double *__restrict__ x,
+				  const double *__restrict__ hv, double t)
+{
+	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const int2 ef = bface[k / 6];
+	const long ipgL = 10L * ef.x + c_face2node[ef.y][k % 6];
+	double fL[6][NV], fR[6][NV], wI[6], ct[9], vn[3];
+	for (int iw = 0; iw < 6; iw++)
+		for (int v = 0; v < NV; v++)
+			fL[iw][v] = (double)kin[kin_at(ne, neq, NV, ipgL, iw, v)];
+	for (int j = 0; j < 9; j++)
+		ct[j] = geom[10L * ef.x + j];
+	face_normal(ct, ef.y, vn);
+	if (hv) {
+		for (int q = 0; q < 6; q++)
+			wI[q] = hv[6 * k + q];
+	} else {
+		const double xx[3] = { x[3 * k], x[3 * k + 1], x[3 * k + 2] };
+		imposed(ip, xx, t, wI);
+	}
+	fr_projector<NV>(kp, fL, vn, wI, fR);
+	for (int iw = 0; iw < 6; iw++)
+		for (int v = 0; v < NV; v++)
+			kin[kin_at(ne, neq, NV, 10L * ne + k, iw, v)] = (T)fR[iw][v];
+}
+
+template <typename T> static void ghost_projector_launch(gdh_ctx *c, const double *hv, double t)
+{
+	const long n = 6L * c->nb;
+	T *kin = (T *)c->w[c->cur];
+	if (c->kp.nv == 4)
+		k_ghost_projector<T, 4><<<nblocks(n, 128), 128, 0, c->stream>>>(
+			kin, c->kp, c->ip, c->ne, c->neq, n, c->bface, c->geom, c->xghost, hv, t);
+	else if (c->kp.nv == 6)
+		k_ghost_projector<T, 6><<<nblocks(n, 128), 128, 0, c->stream>>>(
+			kin, c->kp, c->ip, c->ne, c->neq, n, c->bface, c->geom, c->xghost, hv, t);
+	else {
+		fprintf(stderr, "gdon3d-hip: no projector boundary state for %d velocities\n",
+			c->kp.nv);
+		abort();
+	}
+	HIPCHECK(hipGetLastError());
+}
+
 extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 {
 	const long n = 6L * c->nb;
@@ -573,6 +646,18 @@ extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 		abort();
 	}
 	const double *hv = stage_hostvals(c, wghost, 6 * n);
+	if (c->scheme && c->kp.fr_kind == GDH_FR_PROJECTOR) {
+		/* the projector state reads the element values next to the face,
+		 * which must hold the last step's relaxation */
+		relax_materialize(c);
+		prof_begin(c, 2);
+		if (c->fp32)
+			ghost_projector_launch<float>(c, hv, t);
+		else
+			ghost_projector_launch<double>(c, hv, t);
+		prof_end(c, 2);
+		return;
+	}
 	prof_begin(c, 2);
 	fill_launch(c, 10L * c->ne, n, c->xghost, hv, t, 0.0);
 	prof_end(c, 2);
@@ -581,6 +666,10 @@ extern "C" void gdh_update_boundary(gdh_ctx *c, double t, const double *wghost)
 /* batches: ghosts at each instance's clock + cstage * its time step */
 void ghost_fill_batch(gdh_ctx *c, double cstage)
 {
+	if (c->scheme && c->kp.fr_kind == GDH_FR_PROJECTOR) {
+		fprintf(stderr, "gdon3d-hip: batches take the f_eq boundary state only\n");
+		abort();
+	}
 	const long n = 6L * c->nb;
 	if (n)
 		fill_launch(c, 10L * c->ne, n, c->xghost, nullptr, 0.0, cstage);

@@ -216,6 +216,11 @@ struct KinPar {
 	int nv;
 	double vi[3 * GDH_MAX_V];
 	double lambda, omega;
+	/* boundary state: GDH_FR_FEQ f_eq of the imposed data, GDH_FR_PROJECTOR
+	 * the projector form, with the model's projectors P = c0 I + c1 A^2 + c2 A
+	 * (A = A(n / |n|)): pos (P^{+,0} or the like) and neg */
+	int fr_kind;
+	double proj_pos[3], proj_neg[3];
 };
 
 /* Physical flux q_k = A(e_k) w of Maxwell (the central part of the upwind
@@ -246,6 +251,97 @@ __device__ inline double feq_comp(const KinPar &kp, const double w[6],
 	const int k = v % 3;
 	const double a = w[iw] / 6., b = q[iw][k] / (2. * kp.lambda);
 	return v < 3 ? a + b : a - b;
+}
+
+/* P w = c0 w + c1 A^2 w + c2 A w with A = A(n/|n|): A w = (-m x H, m x E),
+ * A^2 w = ((I - m m^T) E, (I - m m^T) H) (the callbacks
+ * maxwell_get_proj_*_prod_w, src/models/maxwell.c:234-575) */
+__device__ inline void maxwell_proj(const double w[6], const double n[3], const double c[3],
+				    double r[6])
+{
+	const double nrm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+	const double m[3] = { n[0] / nrm, n[1] / nrm, n[2] / nrm };
+	const double *E = w, *H = w + 3;
+	const double aw[6] = { -(m[1] * H[2] - m[2] * H[1]), -(m[2] * H[0] - m[0] * H[2]),
+			       -(m[0] * H[1] - m[1] * H[0]), m[1] * E[2] - m[2] * E[1],
+			       m[2] * E[0] - m[0] * E[2], m[0] * E[1] - m[1] * E[0] };
+	double a2w[6];
+	for (int blk = 0; blk < 2; blk++) {
+		const double *u = w + 3 * blk;
+		const double mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+		for (int d = 0; d < 3; d++)
+			a2w[3 * blk + d] = u[d] - m[d] * mu;
+	}
+	for (int q = 0; q < 6; q++)
+		r[q] = c[0] * w[q] + c[1] * a2w[q] + c[2] * aw[q];
+}
+
+/*
+ * Projector boundary state (the reference's get_fR with use_feq = false,
+ * src/models/d3q4.c:303-332, src/models/d3q6.c:358-379): (wL, yL) = Q(fL)
+ * with y = q - qM(w) the distance of the velocity moments q to the physical
+ * flux; wR = P^-(vn) wI (D3Q4: the reference keeps only that term) or
+ * P^{+,0}(vn) wL + P^-(vn) wI (D3Q6); fR = Q^-1(wR, yL) = M^-1(wR, yL +
+ * qM(wR)). fL[iw][v], fR[iw][v]; NV = kp.nv.
+ */
+template <int NV>
+__device__ inline void fr_projector(const KinPar &kp, const double (&fL)[6][NV],
+				    const double vn[3], const double wI[6], double (&fR)[6][NV])
+{
+	double wL[6], y[6][5];
+	for (int iw = 0; iw < 6; iw++) {
+		double s = 0, r[5] = { 0, 0, 0, 0, 0 };
+		for (int v = 0; v < NV; v++) {
+			const double *vv = kp.vi + 3 * v;
+			const double f = fL[iw][v];
+			s += f;
+			r[0] += f * vv[0];
+			r[1] += f * vv[1];
+			r[2] += f * vv[2];
+			r[3] += f * (vv[0] * vv[0] - vv[1] * vv[1]);
+			r[4] += f * (vv[0] * vv[0] - vv[2] * vv[2]);
+		}
+		wL[iw] = s;
+		for (int k = 0; k < 5; k++)
+			y[iw][k] = r[k];
+	}
+	double qL[6][3];
+	maxwell_qM(wL, qL);
+	for (int iw = 0; iw < 6; iw++)
+		for (int k = 0; k < 3; k++)
+			y[iw][k] -= qL[iw][k];
+	double wR[6];
+	maxwell_proj(wI, vn, kp.proj_neg, wR);
+	if (kp.kind == GDH_KIN_D3Q6) {
+		double pp[6];
+		maxwell_proj(wL, vn, kp.proj_pos, pp);
+		for (int q = 0; q < 6; q++)
+			wR[q] = pp[q] + wR[q];
+	}
+	double qR[6][3];
+	maxwell_qM(wR, qR);
+	const double l = kp.lambda, l2 = l * l;
+	for (int iw = 0; iw < 6; iw++) {
+		const double q0 = y[iw][0] + qR[iw][0], q1 = y[iw][1] + qR[iw][1],
+			     q2 = y[iw][2] + qR[iw][2];
+		if (kp.kind == GDH_KIN_D3Q4) {
+			for (int v = 0; v < NV; v++) {
+				const double *vv = kp.vi + 3 * v;
+				const double qv = q0 * vv[0] + q1 * vv[1] + q2 * vv[2];
+				fR[iw][v] = (wR[iw] + qv / l2) / NV;
+			}
+		} else {
+			/* v = s lambda e_k at index k (s = +1) or 3 + k (s = -1) */
+			const double qk[3] = { q0, q1, q2 };
+			const double c3 = y[iw][3] / (6. * l2), c4 = y[iw][4] / (6. * l2);
+			const double a[3] = { 1, -2, 1 }, b[3] = { 1, 1, -2 };
+			for (int v = 0; v < NV; v++) {
+				const int k = v % 3;
+				const double b0 = wR[iw] / 6., b1 = qk[k] / (2. * l);
+				fR[iw][v] = (v < 3 ? b0 + b1 : b0 - b1) + a[k] * c3 + b[k] * c4;
+			}
+		}
+	}
 }
 
 } // namespace gdh

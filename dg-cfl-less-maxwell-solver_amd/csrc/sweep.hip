@@ -1852,7 +1852,10 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 static void relax_choose(gdh_ctx *c)
 {
 	const char *env = getenv("GDN_RELAX_FUSED");
-	const int fz_only = (c->nlagged || (env && !strcmp(env, "0"))) ? 0
+	/* lagged faces and the projector boundary state read stored element
+	 * values, which must hold the relaxation: no fused form */
+	const int fz_only = (c->nlagged || c->kp.fr_kind == GDH_FR_PROJECTOR ||
+			     (env && !strcmp(env, "0"))) ? 0
 			    : (env && !strcmp(env, "1"))		    ? 1
 									    : -1;
 	int cand[3][2], nc = 0; /* (sweep mode, fused) */

@@ -267,7 +267,10 @@ class Model:
     """A Maxwell gdn_model using the library's callbacks."""
 
     def __init__(self, imposed="plane_wave", kinetic=None, boundary="upwind",
-                 callback=None, lambda_=1.0):
+                 callback=None, lambda_=1.0, boundary_state="feq"):
+        """boundary_state (kinetic): "feq", f_eq of the imposed data (the
+        reference's get_fR), or "projector", the state of get_fR's
+        use_feq = false branch (d3q4_get_fR_projector / d3q6_get_fR_projector)."""
         lib = load_library()
         self.s = GdnModel()
         self.s.nb_w = 6
@@ -286,6 +289,8 @@ class Model:
         if kinetic:
             self.s.use_relax_scheme = True
             getattr(lib, kinetic + "_set_model")(C.byref(self.s))
+            if boundary_state == "projector":
+                self.s.get_fR = fnptr(kinetic + "_get_fR_projector")
         else:
             self.s.use_relax_scheme = False
 

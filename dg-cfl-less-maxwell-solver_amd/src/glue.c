@@ -110,6 +110,27 @@ static int flux_kind(gdn_get_num_flux f)
 	return 0;
 }
 
+/* the library's Maxwell projectors as P = c[0] I + c[1] A^2 + c[2] A
+ * (src/maxwell.c); 0 for any other function */
+static int proj_coefs(gdn_get_proj_neg f, double c[3])
+{
+	static const struct {
+		gdn_get_proj_neg f;
+		double c[3];
+	} known[] = {
+		{ maxwell_get_proj_neg_prod_w, { 0.0, 0.5, -0.5 } },
+		{ maxwell_get_proj_neg_null_prod_w, { 1.0, -0.5, -0.5 } },
+		{ maxwell_get_proj_pos_prod_w, { 0.0, 0.5, 0.5 } },
+		{ maxwell_get_proj_pos_null_prod_w, { 1.0, -0.5, 0.5 } },
+	};
+	for (size_t k = 0; k < sizeof(known) / sizeof(known[0]); k++)
+		if (f == known[k].f) {
+			memcpy(c, known[k].c, sizeof(known[k].c));
+			return 1;
+		}
+	return 0;
+}
+
 void gdn_model_describe(const gdn_model *mdl, int kinetic, gdh_model_desc *out)
 {
 	memset(out, 0, sizeof(*out));
@@ -156,13 +177,24 @@ void gdn_model_describe(const gdn_model *mdl, int kinetic, gdh_model_desc *out)
 	out->kinetic_kind = GDH_KIN_NONE;
 	if (kinetic) {
 		if (mdl->get_w == d3q4_get_w && mdl->get_feq == d3q4_get_feq &&
-		    mdl->get_fR == d3q4_get_fR)
+		    (mdl->get_fR == d3q4_get_fR || mdl->get_fR == d3q4_get_fR_projector))
 			out->kinetic_kind = GDH_KIN_D3Q4;
 		else if (mdl->get_w == d3q6_get_w && mdl->get_feq == d3q6_get_feq &&
-			 mdl->get_fR == d3q6_get_fR)
+			 (mdl->get_fR == d3q6_get_fR || mdl->get_fR == d3q6_get_fR_projector))
 			out->kinetic_kind = GDH_KIN_D3Q6;
 		else
 			gdn_fatal("kinetic callbacks must be the d3q4_* or d3q6_* set");
+		out->fr_kind = (mdl->get_fR == d3q4_get_fR_projector ||
+				mdl->get_fR == d3q6_get_fR_projector)
+				       ? GDH_FR_PROJECTOR
+				       : GDH_FR_FEQ;
+		if (out->fr_kind == GDH_FR_PROJECTOR) {
+			if (!proj_coefs(mdl->get_proj_neg, out->proj_neg) ||
+			    (out->kinetic_kind == GDH_KIN_D3Q6 &&
+			     !proj_coefs((gdn_get_proj_neg)mdl->get_proj_pos, out->proj_pos)))
+				gdn_fatal("the projector boundary state needs get_proj_pos / "
+					  "get_proj_neg among the maxwell_get_proj_*_prod_w");
+		}
 		if (mdl->nb_v > GDH_MAX_V || !mdl->vi)
 			gdn_fatal("invalid velocity set");
 		out->nb_v = mdl->nb_v;

@@ -43,6 +43,12 @@ enum gdh_kinetic_kind {
 	GDH_KIN_D3Q6 = 2, /* d3q6_get_w / d3q6_get_feq / d3q6_get_fR */
 };
 
+/* Boundary state of the kinetic scheme (the model's get_fR) */
+enum gdh_fr_kind {
+	GDH_FR_FEQ = 0, /* d3q4_get_fR / d3q6_get_fR: f_eq of the imposed data */
+	GDH_FR_PROJECTOR = 1, /* d3q4_get_fR_projector / d3q6_get_fR_projector */
+};
+
 #define GDH_MAX_V 8
 #define GDH_IMPOSED_NPAR 16
 
@@ -77,6 +83,12 @@ typedef struct gdh_model_desc {
 	double vi[3 * GDH_MAX_V];
 	double lambda;
 	double omega;
+	/* boundary state (GDH_FR_*) and, for GDH_FR_PROJECTOR, the model's
+	 * get_proj_pos / get_proj_neg as P = c[0] I + c[1] A^2 + c[2] A with
+	 * A = A(n / |n|) */
+	int fr_kind;
+	double proj_pos[3];
+	double proj_neg[3];
 } gdh_model_desc;
 
 typedef struct gdh_ctx gdh_ctx;

@@ -840,6 +840,132 @@ static void kin_feq(const okin *k, const double *w, double *feq)
 	}
 }
 
+/* Maxwell eigen-projectors (src/models/maxwell.c:234-575) from the 6x6
+ * matrix A(m), m = n / |n|, A w = (-m x H, m x E): P^- = (A^2 - A) / 2 onto
+ * the incoming characteristics, P^{+,0} = I - P^- */
+static void proj_matrices(const double *n, double pneg[6][6], double ppos[6][6])
+{
+	const double r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+	const double m[3] = { n[0] / r, n[1] / r, n[2] / r };
+	const double N[3][3] = { { 0, -m[2], m[1] }, { m[2], 0, -m[0] }, { -m[1], m[0], 0 } };
+	double A[6][6] = { { 0 } }, A2[6][6];
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) {
+			A[i][3 + j] = -N[i][j];
+			A[3 + i][j] = N[i][j];
+		}
+	for (int i = 0; i < 6; i++)
+		for (int j = 0; j < 6; j++) {
+			A2[i][j] = 0;
+			for (int k = 0; k < 6; k++)
+				A2[i][j] += A[i][k] * A[k][j];
+		}
+	for (int i = 0; i < 6; i++)
+		for (int j = 0; j < 6; j++) {
+			pneg[i][j] = 0.5 * (A2[i][j] - A[i][j]);
+			ppos[i][j] = (i == j ? 1.0 : 0.0) - pneg[i][j];
+		}
+}
+
+void orc_proj(const double *w, const double *n, int neg, double *res)
+{
+	double pn[6][6], pp[6][6];
+	proj_matrices(n, pn, pp);
+	for (int i = 0; i < 6; i++) {
+		res[i] = 0;
+		for (int j = 0; j < 6; j++)
+			res[i] += (neg ? pn[i][j] : pp[i][j]) * w[j];
+	}
+}
+
+/* the velocity moments of f (src/models/d3q4.c:105-135,
+ * src/models/d3q6.c:100-141): w, then per field nm = nv - 1 moments
+ * X, Y, Z (D3Q6 also X^2 - Y^2, X^2 - Z^2) in q[nm iw + k] */
+static void kin_moments(const okin *k, const double *f, double *w, double *q)
+{
+	const int nm = k->nv - 1;
+	for (int iw = 0; iw < 6; iw++) {
+		w[iw] = 0;
+		for (int c = 0; c < nm; c++)
+			q[nm * iw + c] = 0;
+		for (int iv = 0; iv < k->nv; iv++) {
+			const double fi = f[iw * k->nv + iv], *v = k->vi + 3 * iv;
+			w[iw] += fi;
+			for (int d = 0; d < 3; d++)
+				q[nm * iw + d] += fi * v[d];
+			if (nm == 5) {
+				q[nm * iw + 3] += fi * (v[0] * v[0] - v[1] * v[1]);
+				q[nm * iw + 4] += fi * (v[0] * v[0] - v[2] * v[2]);
+			}
+		}
+	}
+}
+
+/* the physical flux in moments: A(e_d) w for X, Y, Z, 0 beyond */
+static void kin_qM(const okin *k, const double *w, double *qM)
+{
+	const int nm = k->nv - 1;
+	const double e[3][3] = { { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+	double fl[3][6];
+	for (int d = 0; d < 3; d++)
+		flux_upwind(w, w, e[d], fl[d]);
+	for (int iw = 0; iw < 6; iw++)
+		for (int c = 0; c < nm; c++)
+			qM[nm * iw + c] = c < 3 ? fl[c][iw] : 0;
+}
+
+/* f from its moments (src/models/d3q4.c:137-157, src/models/d3q6.c:143-208) */
+static void kin_from_moments(const okin *k, const double *w, const double *q, double *f)
+{
+	const double l = k->lambda, l2 = l * l;
+	for (int iw = 0; iw < 6; iw++) {
+		if (k->nv == 4) {
+			for (int iv = 0; iv < 4; iv++)
+				f[iw * 4 + iv] = (w[iw] + (q[3 * iw] * k->vi[3 * iv] +
+							   q[3 * iw + 1] * k->vi[3 * iv + 1] +
+							   q[3 * iw + 2] * k->vi[3 * iv + 2]) / l2) / 4;
+		} else {
+			const double *r = q + 5 * iw;
+			f[iw * 6 + 0] = +w[iw] / 6. + r[0] / (2. * l) + r[3] / (6. * l2) + r[4] / (6. * l2);
+			f[iw * 6 + 1] = +w[iw] / 6. + r[1] / (2. * l) - r[3] / (3. * l2) + r[4] / (6. * l2);
+			f[iw * 6 + 2] = +w[iw] / 6. + r[2] / (2. * l) + r[3] / (6. * l2) - r[4] / (3. * l2);
+			f[iw * 6 + 3] = +w[iw] / 6. - r[0] / (2. * l) + r[3] / (6. * l2) + r[4] / (6. * l2);
+			f[iw * 6 + 4] = +w[iw] / 6. - r[1] / (2. * l) - r[3] / (3. * l2) + r[4] / (6. * l2);
+			f[iw * 6 + 5] = +w[iw] / 6. - r[2] / (2. * l) + r[3] / (6. * l2) - r[4] / (3. * l2);
+		}
+	}
+}
+
+/* The boundary state of get_fR's use_feq = false branch
+ * (src/models/d3q4.c:303-332, src/models/d3q6.c:358-379) with the model
+ * projectors of the reference examples (get_proj_pos =
+ * maxwell_get_proj_pos_null_prod_w, get_proj_neg =
+ * maxwell_get_proj_neg_prod_w): (wL, yL) = Q(fL); wR = P^- wI (D3Q4; the
+ * reference overwrites the sum with that term) or P^{+,0} wL + P^- wI (D3Q6);
+ * fR = Q^-1(wR, yL). */
+void orc_fr_projector(int kind, const double *fL, const double *vn, const double *wI,
+		      double *fR)
+{
+	okin k;
+	kin_set(&k, kind);
+	const int nm = k.nv - 1;
+	double wL[6], q[30], qM[30], wR[6], pn[6][6], pp[6][6];
+	kin_moments(&k, fL, wL, q);
+	kin_qM(&k, wL, qM);
+	for (int c = 0; c < 6 * nm; c++)
+		q[c] -= qM[c]; /* yL */
+	proj_matrices(vn, pn, pp);
+	for (int i = 0; i < 6; i++) {
+		wR[i] = 0;
+		for (int j = 0; j < 6; j++)
+			wR[i] += pn[i][j] * wI[j] + (kind == 2 ? pp[i][j] * wL[j] : 0.0);
+	}
+	kin_qM(&k, wR, qM);
+	for (int c = 0; c < 6 * nm; c++)
+		q[c] += qM[c];
+	kin_from_moments(&k, wR, q, fR);
+}
+
 /* =================================================================== */
 /*                          sparse matrices                            */
 /* =================================================================== */
@@ -1204,6 +1330,7 @@ typedef struct {
 	double *wn;
 	double t_asm, t_run;
 	int nthreads;
+	int fr_projector; /* boundary state: 0 f_eq(wI), 1 the projector form */
 	/* assembled operators */
 	csr Minv, F; /* RK3 */
 	csr M1[8], M2[8];
@@ -1242,6 +1369,11 @@ void *orc_sim_create(void *mesh, int scheme, int imposed_kind,
 void orc_sim_set_lambda(void *h, double lambda)
 {
 	((osim *)h)->lambda = lambda;
+}
+
+void orc_sim_set_fr_projector(void *h, int on)
+{
+	((osim *)h)->fr_projector = on;
 }
 
 void orc_sim_set_threads(void *h, int n)
@@ -1308,6 +1440,8 @@ static void init_sol(osim *s)
 }
 
 /* src/dg/dg.c:61-107 */
+static void face_vn(const omesh *m, int F, double vn[3]);
+
 static void update_boundary(osim *s, double t)
 {
 	const omesh *m = s->m;
@@ -1319,7 +1453,16 @@ static void update_boundary(osim *s, double t)
 			int ipgR = NQN * m->nelem + NQV * bf + a;
 			const double *x = m->xq + 3 * m->e2q[ipgL];
 			exact(&s->ip, x, t, w);
-			if (s->scheme) {
+			if (s->scheme && s->fr_projector) {
+				/* src/dg/dg.c:95-100: fL at the L node, the face's normal */
+				double fl[48], vn[3];
+				for (int im = 0; im < s->mm; im++)
+					fl[im] = s->wn[kidx(s, im, ipgL)];
+				face_vn(m, F, vn);
+				orc_fr_projector(s->kin.kind, fl, vn, w, f);
+				for (int im = 0; im < s->mm; im++)
+					s->wn[kidx(s, im, ipgR)] = f[im];
+			} else if (s->scheme) {
 				kin_feq(&s->kin, w, f);
 				for (int im = 0; im < s->mm; im++)
 					s->wn[kidx(s, im, ipgR)] = f[im];

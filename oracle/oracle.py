@@ -57,6 +57,9 @@ def lib():
             "orc_init_solution": (None, [C.c_void_p]),
             "orc_error_l2": (C.c_double, [C.c_void_p]),
             "orc_energy": (C.c_double, [C.c_void_p]),
+            "orc_sim_set_fr_projector": (None, [C.c_void_p, C.c_int]),
+            "orc_proj": (None, [dp, dp, C.c_int, dp]),
+            "orc_fr_projector": (None, [C.c_int, dp, dp, dp, dp]),
             "orc_sim_free": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -115,7 +118,7 @@ class OSim:
     """Assembled-matrix solve of the reference algorithm on an OMesh."""
 
     def __init__(self, mesh, scheme, imposed_kind, imposed_par=None, kin=0,
-                 bflux=UPWIND, lam=None, threads=None):
+                 bflux=UPWIND, lam=None, threads=None, fr_projector=False):
         par = np.zeros(16)
         if imposed_par is not None:
             par[: len(imposed_par)] = imposed_par
@@ -124,6 +127,7 @@ class OSim:
         if lam is not None:
             lib().orc_sim_set_lambda(self.h, lam)
         lib().orc_sim_set_threads(self.h, threads or os.cpu_count() or 1)
+        lib().orc_sim_set_fr_projector(self.h, int(fr_projector))
         self.info()
 
     def info(self):
@@ -198,6 +202,22 @@ class OSim:
         if self.h:
             lib().orc_sim_free(self.h)
             self.h = None
+
+
+def proj(w, n, neg):
+    """the oracle's Maxwell eigen-projector P^- (neg) or P^{+,0} applied to w"""
+    w, n = np.ascontiguousarray(w, dtype=np.float64), np.ascontiguousarray(n, dtype=np.float64)
+    out = np.zeros(6)
+    lib().orc_proj(_d(w), _d(n), int(neg), _d(out))
+    return out
+
+
+def fr_projector(kind, fL, vn, wI):
+    """the oracle's projector boundary state (kind D3Q4 or D3Q6)"""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (fL, vn, wI)]
+    out = np.zeros(len(a[0]))
+    lib().orc_fr_projector(kind, _d(a[0]), _d(a[1]), _d(a[2]), _d(out))
+    return out
 
 
 # ------------------------------------------------------------- reference

@@ -256,3 +256,30 @@ def test_reference_example_builds_against_include(name, tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert exe.exists()
+
+
+@needs_ref
+@pytest.mark.parametrize("kin", ["d3q4", "d3q6"])
+def test_oracle_projector_boundary_state_matches_reference(kin):
+    """The oracle's restatement (6x6 eigen-projectors built from A(n) and
+    its own moment transforms) against the reference's projectors and its
+    apply_Q / apply_Q_inv composed as get_fR's use_feq = false branch."""
+    R = O.ref()
+    nv, nm = NM[kin]
+    mr = model(R, kin)
+    for _ in range(50):
+        w, n = rng.normal(size=6), rng.normal(size=3)
+        assert np.allclose(O.proj(w, n, True), proj(R, "maxwell_get_proj_neg_prod_w", w, n),
+                           rtol=1e-13, atol=1e-13)
+        assert np.allclose(O.proj(w, n, False),
+                           proj(R, "maxwell_get_proj_pos_null_prod_w", w, n),
+                           rtol=1e-13, atol=1e-13)
+        fL, wI, vn = rng.normal(size=6 * nv), rng.normal(size=6), rng.normal(size=3)
+        wL, yL = np.zeros(6), np.zeros(6 * nm)
+        call(R, kin + "_apply_Q", mr, fL, wL, yL)
+        pn = proj(R, "maxwell_get_proj_neg_prod_w", wI, vn)
+        wR = pn if kin == "d3q4" else proj(R, "maxwell_get_proj_pos_null_prod_w", wL, vn) + pn
+        expect = np.zeros(6 * nv)
+        call(R, kin + "_apply_Q_inv", mr, wR, yL, expect)
+        got = O.fr_projector(O.D3Q4 if kin == "d3q4" else O.D3Q6, fL, vn, wI)
+        assert np.allclose(got, expect, rtol=1e-13, atol=1e-13)

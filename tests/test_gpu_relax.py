@@ -307,3 +307,56 @@ def test_smallest_meshes_every_sweep_form(n, form, monkeypatch):
     G.set_cavity_mode(1, 1, 1, (1, -1, 0))
     check_form(path, "cavity_mode", O.CAVITY_MODE, cavity_par(), "d3q4", 0.5, 5.0, form,
                monkeypatch)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("form", [("item", "0"), ("cluster", "0"), ("level", "0")],
+                         ids=["item", "cluster", "level"])
+@pytest.mark.parametrize("kin", ["d3q4", "d3q6"])
+def test_relax_projector_boundary_state(kin, form, precision, monkeypatch):
+    """The boundary state of the reference's get_fR with use_feq = false
+    (src/models/d3q4.c:303-332, src/models/d3q6.c:358-379), selected by
+    d3qN_get_fR_projector and computed on the device from the relaxed state
+    next to each boundary face, against the oracle's restatement: relative
+    L2 <= 1e-12 in binary64 (<= 2e-5 in binary32) on the full micro state
+    after whole solves, every sweep form (the relaxation is never fused
+    into the sweep here: the ghost update reads the relaxed values)."""
+    path = kuhn_msh(4, 0.15, 19)
+    k = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
+    G.set_plane_wave(k, (0, 0, 1))
+    sweep, fused = form
+    monkeypatch.setenv("GDN_SWEEP", sweep)
+    monkeypatch.setenv("GDN_RELAX_FUSED", fused)
+    mesh = G.Mesh(path=path)
+    sim = G.Simulation(mesh, G.Model("plane_wave", kinetic=kin, boundary_state="projector"))
+    sim.set_time(0.3, 0.0, 5.0)
+    st = G.Stepper(sim, precision=precision)
+    ran = (G.sweep_info(st)["form"], str(G.relax_fused(st)))
+    st.advance(sim.s.itermax)
+    st.finish()
+    wn = sim.wn()
+    st.free()
+    sim.free()
+    om = O.OMesh(path)
+    s = O.OSim(om, 1, O.PLANE_WAVE, plane_wave_par(k, (0, 0, 1)),
+               kin=O.D3Q4 if kin == "d3q4" else O.D3Q6, fr_projector=True)
+    s.set_time(0.3, 0.0, 5.0)
+    s.solve()
+    ref = s.wn()
+    # the f_eq boundary state gives another solution: the projector one is
+    # what was run
+    s2 = O.OSim(om, 1, O.PLANE_WAVE, plane_wave_par(k, (0, 0, 1)),
+                kin=O.D3Q4 if kin == "d3q4" else O.D3Q6)
+    s2.set_time(0.3, 0.0, 5.0)
+    s2.solve()
+    other = s2.wn()
+    s.free()
+    s2.free()
+    om.free()
+    r = rel_l2(wn, ref)
+    print(f"projector boundary {kin} {form} {precision}: ran {ran}, rel L2 vs oracle {r:.3e}, "
+          f"f_eq-boundary oracle differs by {rel_l2(other, ref):.3e}", flush=True)
+    assert ran == (sweep, "0")
+    assert rel_l2(other, ref) > 1e-6
+    assert r <= (TOL if precision == "fp64" else 2e-5)
