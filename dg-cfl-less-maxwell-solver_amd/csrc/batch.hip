@@ -35,10 +35,7 @@ extern "C" void gdh_batch_setup(gdh_ctx *c, int ninst, const int *elem_first,
 		fprintf(stderr, "gdon3d-hip: batches need closed-form exact data\n");
 		abort();
 	}
-	if (eps && c->scheme) {
-		fprintf(stderr, "gdon3d-hip: permittivity is supported for RK3 only\n");
-		abort();
-	}
+
 	const int ne = c->ne, nb = c->nb;
 	std::vector<int> einst(ne), binst(nb > 0 ? nb : 1, 0), ecol(ne);
 	for (int i = 0; i < ninst; i++) {
@@ -47,18 +44,12 @@ extern "C" void gdh_batch_setup(gdh_ctx *c, int ninst, const int *elem_first,
 		for (int b = bface_first[i]; b < bface_first[i + 1]; b++)
 			binst[b] = i;
 	}
-	std::vector<double> ieps;
 	if (!c->scheme) {
 		std::vector<int> pos(ne);
 		HIPCHECK(hipMemcpy(pos.data(), c->elem_pos, sizeof(int) * ne,
 				   hipMemcpyDeviceToHost));
 		for (int e = 0; e < ne; e++)
 			ecol[pos[e]] = einst[e];
-		if (eps) {
-			ieps.resize(ne);
-			for (int e = 0; e < ne; e++)
-				ieps[pos[e]] = 1.0 / eps[e];
-		}
 	}
 	c->bt_mem[0] = (int *)dmalloc(sizeof(int) * ne);
 	c->bt_mem[1] = (int *)dmalloc(sizeof(int) * binst.size());
@@ -79,11 +70,6 @@ extern "C" void gdh_batch_setup(gdh_ctx *c, int ninst, const int *elem_first,
 			   hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(c->bt_dmem[1], dt, sizeof(double) * ninst, hipMemcpyHostToDevice));
 	HIPCHECK(hipMemset(c->bt_dmem[2], 0, sizeof(double) * ninst));
-	if (!ieps.empty()) {
-		c->bt_dmem[3] = (double *)dmalloc(sizeof(double) * ne);
-		HIPCHECK(hipMemcpy(c->bt_dmem[3], ieps.data(), sizeof(double) * ne,
-				   hipMemcpyHostToDevice));
-	}
 	c->bt.ninst = ninst;
 	c->bt.einst = c->bt_mem[0];
 	c->bt.binst = c->bt_mem[1];
@@ -91,7 +77,10 @@ extern "C" void gdh_batch_setup(gdh_ctx *c, int ninst, const int *elem_first,
 	c->bt.par = c->bt_dmem[0];
 	c->bt.dt = c->bt_dmem[1];
 	c->bt.time = c->bt_dmem[2];
-	c->bt.ieps_col = c->bt_dmem[3];
+	/* permittivity per element: RK3 per storage column, kinetic per
+	 * element (the equilibrium's flux on (D / eps, B)) */
+	if (eps)
+		set_permittivity(c, eps);
 	c->bt_e0.assign(elem_first, elem_first + ninst + 1);
 	c->bt_nb0.assign(bface_first, bface_first + ninst + 1);
 	if (c->scheme) {

@@ -59,6 +59,8 @@ struct gdh_ctx {
 	double *geom;
 	int4 *nbr, *nbmap;
 	int2 *bface; /* kinetic: per boundary face, its element and local face */
+	double *ieps; /* 1/eps per element (kinetic: mesh order; RK3: by storage
+		       * column), or null */
 	double *xnode, *xghost;
 	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
 	 * GH[q][6 bf + a]; kinetic: w = item-major state (kin_at), ghosts
@@ -271,5 +273,7 @@ void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl, int relax_in);
 void relax_materialize(gdh_ctx *c);
 void relax_launch(gdh_ctx *c, void *f, long n);
 void block_inverse_launch(gdh_ctx *c);
+void set_permittivity(gdh_ctx *c, const double *eps);
+void check_sweep_status(gdh_ctx *c);
 
 #endif

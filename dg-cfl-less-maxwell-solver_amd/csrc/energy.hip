@@ -23,6 +23,10 @@ struct EnergyRef {
 	const double *wn;
 	int nv;
 	long neq;
+	__device__ double weight_e(long) const
+	{
+		return 1.0;
+	}
 	__device__ double w(long ipg, int q) const
 	{
 		if (!nv)
@@ -41,6 +45,7 @@ template <typename T> struct EnergyDev {
 	StateRef<T> sr;
 	int nv; /* 0: RK3 */
 	long neq;
+	const double *ieps; /* 1/eps (kinetic: by element; RK3: by column) or null */
 	__device__ double w(long ipg, int q) const
 	{
 		if (!nv)
@@ -49,6 +54,14 @@ template <typename T> struct EnergyDev {
 		for (int v = 0; v < nv; v++)
 			s += (double)kin[kin_at(sr.ne, neq, nv, ipg, q, v)];
 		return s;
+	}
+	/* weight of the E (D) components of element e: eps |E|^2 (RK3),
+	 * |D|^2 / eps (kinetic) */
+	__device__ double weight_e(long e) const
+	{
+		if (!ieps)
+			return 1.0;
+		return nv ? ieps[e] : 1.0 / ieps[sr.pos[e]];
 	}
 };
 
@@ -63,8 +76,12 @@ __global__ __launch_bounds__(EN_BS) void k_energy_partial(A a, const double *__r
 	double acc = 0.0;
 	for (long e = (long)blockIdx.x * EN_BS + threadIdx.x; e < ne;
 	     e += (long)gridDim.x * EN_BS) {
-		double s = 0.0;
+		double s = 0.0, se = 0.0;
 		for (int q = 0; q < 6; q++) {
+			if (q == 3) { /* E components done: their weight */
+				se = s;
+				s = 0.0;
+			}
 			double w[10];
 #pragma unroll
 			for (int i = 0; i < 10; i++)
@@ -78,7 +95,7 @@ __global__ __launch_bounds__(EN_BS) void k_energy_partial(A a, const double *__r
 				s += w[i] * mw;
 			}
 		}
-		acc += det[e * dstride] * s;
+		acc += det[e * dstride] * (a.weight_e(e) * se + s);
 	}
 	red[threadIdx.x] = acc;
 	__syncthreads();
@@ -146,6 +163,7 @@ extern "C" double gdh_energy(gdh_ctx *c)
 		a.sr.ngh = 6L * c->nb;
 		a.nv = c->scheme ? c->kp.nv : 0;
 		a.neq = c->neq;
+		a.ieps = c->ieps;
 		energy_launch(a, c->geom + 9, 10, c->ne, c->partials, out, c->stream);
 	} else {
 		EnergyDev<double> a;
@@ -157,11 +175,13 @@ extern "C" double gdh_energy(gdh_ctx *c)
 		a.sr.ngh = 6L * c->nb;
 		a.nv = c->scheme ? c->kp.nv : 0;
 		a.neq = c->neq;
+		a.ieps = c->ieps;
 		energy_launch(a, c->geom + 9, 10, c->ne, c->partials, out, c->stream);
 	}
 	double e = 0.0;
 	HIPCHECK(hipMemcpyAsync(&e, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
+	check_sweep_status(c);
 	return e;
 }
 

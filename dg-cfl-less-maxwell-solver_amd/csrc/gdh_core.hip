@@ -163,6 +163,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->elem_pos = nullptr;
 	c->stage = nullptr;
 	c->bface = nullptr;
+	c->ieps = nullptr;
 	if (scheme) {
 		/* the sweep reads neighbours in mesh order */
 		HIPCHECK(hipMemcpy(c->nbr, mesh->nbr, sizeof(int4) * ne,
@@ -181,6 +182,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 		c->bface = (int2 *)dmalloc(sizeof(int2) * bf.size());
 		HIPCHECK(hipMemcpy(c->bface, bf.data(), sizeof(int2) * bf.size(),
 				   hipMemcpyHostToDevice));
+		c->kp.bface = c->bface;
 	} else {
 		/* RK3 works on storage columns: geometry, neighbours (as columns)
 		 * and face maps permuted, ghosts unchanged */
@@ -303,7 +305,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev, c->chead, c->cdesc,
 			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
 			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
-			 c->stage, c->bface };
+			 c->stage, c->bface, c->ieps };
 	for (void *p : ptrs)
 		if (p)
 			HIPCHECK(hipFree(p));
@@ -418,11 +420,13 @@ extern "C" void gdh_download_state(gdh_ctx *c, double *wn)
 	HIPCHECK(hipMemcpyAsync(wn, c->stage, sizeof(double) * len, hipMemcpyDeviceToHost,
 				c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
+	check_sweep_status(c);
 }
 
-extern "C" void gdh_synchronize(gdh_ctx *c)
+/* after a stream synchronisation: abort if a sweep's bounded wait expired
+ * (its results are then not a solution) */
+void check_sweep_status(gdh_ctx *c)
 {
-	HIPCHECK(hipStreamSynchronize(c->stream));
 	if (c->status) {
 		int st = 0;
 		HIPCHECK(hipMemcpy(&st, c->status, sizeof(int), hipMemcpyDeviceToHost));
@@ -432,6 +436,50 @@ extern "C" void gdh_synchronize(gdh_ctx *c)
 			abort();
 		}
 	}
+}
+
+extern "C" void gdh_synchronize(gdh_ctx *c)
+{
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	check_sweep_status(c);
+}
+
+/* 1/eps per mesh element into the context: RK3 by storage column
+ * (bt.ieps_col, read by the stage kernels), kinetic in mesh order
+ * (kp.ieps) */
+void set_permittivity(gdh_ctx *c, const double *eps)
+{
+	const long ne = c->ne;
+	std::vector<double> ie(ne > 0 ? ne : 1, 1.0);
+	std::vector<int> pos;
+	if (!c->scheme) {
+		pos.resize(ne);
+		HIPCHECK(hipMemcpy(pos.data(), c->elem_pos, sizeof(int) * ne, hipMemcpyDeviceToHost));
+	}
+	for (long e = 0; e < ne; e++) {
+		if (!(eps[e] > 0.0) || !std::isfinite(eps[e])) {
+			fprintf(stderr, "gdon3d-hip: permittivity of element %ld is %g, not positive\n",
+				e, eps[e]);
+			abort();
+		}
+		ie[c->scheme ? e : pos[e]] = 1.0 / eps[e];
+	}
+	if (c->scheme && c->kp.fr_kind == GDH_FR_PROJECTOR) {
+		fprintf(stderr, "gdon3d-hip: the projector boundary state takes eps = 1 only\n");
+		abort();
+	}
+	if (!c->ieps)
+		c->ieps = (double *)dmalloc(sizeof(double) * ie.size());
+	HIPCHECK(hipMemcpy(c->ieps, ie.data(), sizeof(double) * ie.size(), hipMemcpyHostToDevice));
+	if (c->scheme)
+		c->kp.ieps = c->ieps;
+	else
+		c->bt.ieps_col = c->ieps;
+}
+
+extern "C" void gdh_set_permittivity(gdh_ctx *c, const double *eps)
+{
+	set_permittivity(c, eps);
 }
 
 extern "C" void gdh_rk3_info(gdh_ctx *c, int out[3])
@@ -492,8 +540,9 @@ static const double *stage_hostvals(gdh_ctx *c, const double *h, size_t n)
 /*                   initial condition and ghost data                    */
 /* ===================================================================== */
 
-/* Write the macro state wv (6 values) of DoF ipg into the state, as w
- * itself (RK3, SoA) or as f_eq(w) (kinetic, item-major). */
+/* Write the macro state wv = (E, H) (6 values) of DoF ipg into the state,
+ * as w itself (RK3, SoA) or as f_eq of the conserved fields (eps E, H)
+ * (kinetic, item-major; the flux of the equilibrium is that of (E, H)). */
 template <typename T>
 __device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 				 const KinPar &kp, int neq, long ipg,
@@ -507,10 +556,16 @@ __device__ inline void store_dof(const StateRef<T> &s, T *kin, int scheme,
 	}
 	double qm[6][3];
 	maxwell_qM(wv, qm);
+	double wc[6] = { wv[0], wv[1], wv[2], wv[3], wv[4], wv[5] };
+	if (kp.ieps) {
+		const double ie = kin_ieps(kp, ipg, s.ne);
+		for (int q = 0; q < 3; q++)
+			wc[q] = wv[q] / ie;
+	}
 	for (int iw = 0; iw < 6; iw++)
 		for (int v = 0; v < kp.nv; v++)
 			kin[kin_at(s.ne, neq, kp.nv, ipg, iw, v)] =
-				(T)feq_comp(kp, wv, qm, iw, v);
+				(T)feq_comp(kp, wc, qm, iw, v);
 }
 
 /* t + c dt rounded as the host loops round it (no fused multiply-add) */
@@ -717,6 +772,10 @@ __global__ __launch_bounds__(ERR_BS) void k_err_partial(
 						s += (double)st[kin_at(sr.ne, neq, kp.nv, ipg, q, v)];
 					wl[q] = s;
 				}
+				/* E = D / eps against the exact E */
+				if (kp.ieps)
+					for (int q = 0; q < 3; q++)
+						wl[q] *= kp.ieps[e];
 			}
 			for (int q = 0; q < 6; q++) {
 				const double d = wl[q] - wx[q];
@@ -788,6 +847,7 @@ double error_range(gdh_ctx *c, int e0, int e1, double t, const ImposedPar &ip,
 	HIPCHECK(hipMemcpyAsync(&err, c->partials + 6L * c->npartials,
 				sizeof(double), hipMemcpyDeviceToHost, c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
+	check_sweep_status(c);
 	return err;
 }
 

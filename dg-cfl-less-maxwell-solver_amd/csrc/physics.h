@@ -203,6 +203,14 @@ __device__ inline void imposed(const ImposedPar &ip, const double x[3], double t
 		w[5] = -st * (k[0] * a[1] - k[1] * a[0]) * c[0] * c[1] * s[2];
 		break;
 	}
+	case GDH_IMPOSED_GAUSSIAN_PULSE: {
+		const double d0 = x[0] - ip.p[0], d1 = x[1] - ip.p[1], d2 = x[2] - ip.p[2];
+		const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+		for (int q = 0; q < 6; q++)
+			w[q] = 0.0;
+		w[2] = ip.p[4] * exp(-r2 / (2 * ip.p[3] * ip.p[3]));
+		break;
+	}
 	default:
 		for (int q = 0; q < 6; q++)
 			w[q] = 0.0;
@@ -221,7 +229,23 @@ struct KinPar {
 	 * (A = A(n / |n|)): pos (P^{+,0} or the like) and neg */
 	int fr_kind;
 	double proj_pos[3], proj_neg[3];
+	/* permittivity: 1/eps per mesh element (null: eps = 1 everywhere); the
+	 * kinetic state then carries the conserved fields (D, B) = (eps E, H),
+	 * and the physical flux of the equilibrium is taken on (D / eps, B).
+	 * bface: element and local face of each boundary face (ghost DoFs take
+	 * the permittivity of their element). */
+	const double *ieps;
+	const int2 *bface;
 };
+
+/* 1/eps at DoF ipg (element DoFs, or ghost DoFs through their element) */
+__device__ inline double kin_ieps(const KinPar &kp, long ipg, long ne)
+{
+	if (!kp.ieps)
+		return 1.0;
+	const long e = ipg < 10 * ne ? ipg / 10 : kp.bface[(ipg - 10 * ne) / 6].x;
+	return kp.ieps[e];
+}
 
 /* Physical flux q_k = A(e_k) w of Maxwell (the central part of the upwind
  * flux with n = e_k; its jump terms vanish for wL = wR). q[iw][k]. */

@@ -271,7 +271,7 @@ __device__ inline void maxwell_q_row(const double w[6], int iw, double r[3])
  * 42-54).
  */
 template <int NV, typename R>
-__device__ inline double relax_value(const KinPar &kp, const R (&fr)[NV], int v)
+__device__ inline double relax_value(const KinPar &kp, const R (&fr)[NV], int v, double ie)
 {
 	const int t = threadIdx.x % 64;
 	const int tt = t < 60 ? t : 59, iw = tt / 10, i = tt % 10;
@@ -285,6 +285,10 @@ __device__ inline double relax_value(const KinPar &kp, const R (&fr)[NV], int v)
 #pragma unroll
 	for (int q = 0; q < 6; q++)
 		w[q] = __shfl(s, 10 * q + i, 64);
+	/* the flux is taken on (D / eps, B); ie = 1 exactly without permittivity */
+#pragma unroll
+	for (int q = 0; q < 3; q++)
+		w[q] *= ie;
 	double r[3];
 	maxwell_q_row(w, iw, r);
 	/* the model's scalars taken where they are used (not values derived
@@ -435,7 +439,7 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 				      const SweepTables &T, ItemBuf &B, PrepScratch &W, R &fv)
 {
 	const int t = threadIdx.x;
-	fv = relax_in ? (R)relax_value<NV>(kp, P.fr, d.v) : P.fr[0];
+	fv = relax_in ? (R)relax_value<NV>(kp, P.fr, d.v, kp.ieps ? kp.ieps[d.e] : 1.0) : P.fr[0];
 	const int gf = given_faces(d), sf = slot_faces(d);
 	const int iw = t / 10, i = t % 10;
 	W.D[t / 10][t % 10] = P.d0;
@@ -1444,8 +1448,11 @@ __global__ __launch_bounds__(256) void k_relax_dof(R *__restrict__ f, KinPar kp,
 		}
 		w[iw] = s;
 	}
+	/* the flux of the equilibrium on (D / eps, B) */
+	const double ie = kin_ieps(kp, ipg, ne);
+	const double wt[6] = { w[0] * ie, w[1] * ie, w[2] * ie, w[3], w[4], w[5] };
 	double q[6][3];
-	maxwell_qM(w, q);
+	maxwell_qM(wt, q);
 	const double om = kp.omega;
 #pragma unroll
 	for (int iw = 0; iw < 6; iw++)
@@ -1483,6 +1490,11 @@ __global__ __launch_bounds__(BS) void k_relax(R *__restrict__ f, KinPar kp, long
 #pragma unroll
 	for (int q = 0; q < 6; q++)
 		w[q] = __shfl(s, 10 * q + i, 64);
+	/* the flux of the equilibrium on (D / eps, B) */
+	const double ie = kp.ieps ? kp.ieps[e] : 1.0;
+#pragma unroll
+	for (int q = 0; q < 3; q++)
+		w[q] *= ie;
 	double r[3];
 	maxwell_q_row(w, iw, r);
 	const double om = kp.omega;

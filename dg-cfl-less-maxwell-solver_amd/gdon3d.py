@@ -141,6 +141,9 @@ def load_library(path=LIB_PATH):
         "dg_stepper_create": (C.c_void_p, [P(GdnSimulation)]),
         "dg_stepper_create_precision": (C.c_void_p, [P(GdnSimulation), C.c_int]),
         "dg_stepper_init": (None, [C.c_void_p]),
+        "dg_stepper_set_permittivity": (None, [C.c_void_p, c_double_p]),
+        "maxwell_plane_wave_set_eps": (None, [c_double_p, c_double_p, C.c_double, C.c_double]),
+        "maxwell_gaussian_pulse_set": (None, [c_double_p, C.c_double, C.c_double]),
         "dg_stepper_advance": (None, [C.c_void_p, C.c_int]),
         "dg_stepper_time": (C.c_double, [C.c_void_p]),
         "dg_stepper_iterations": (C.c_int, [C.c_void_p]),
@@ -301,6 +304,18 @@ def set_plane_wave(k, e0, phase=0.0):
     load_library().maxwell_plane_wave_set(dptr(k), dptr(e0), phase)
 
 
+def set_plane_wave_eps(k, e0, eps, phase=0.0):
+    """plane wave of speed 1/sqrt(eps) (exact for a uniform permittivity)"""
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    e0 = np.ascontiguousarray(e0, dtype=np.float64)
+    load_library().maxwell_plane_wave_set_eps(dptr(k), dptr(e0), phase, eps)
+
+
+def set_gaussian_pulse(center=(0.5, 0.5, 0.5), sigma=0.08, amp=1.0):
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    load_library().maxwell_gaussian_pulse_set(dptr(c), sigma, amp)
+
+
 def set_cavity_mode(l, m, n, a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     load_library().maxwell_cavity_mode_set(l, m, n, dptr(a))
@@ -340,13 +355,18 @@ class Stepper:
 
     PRECISIONS = {"fp64": 0, "fp32": 1}
 
-    def __init__(self, sim, precision="fp64"):
+    def __init__(self, sim, precision="fp64", eps=None):
         """precision: "fp64" (the plain form) or "fp32" (binary32 fields,
-        binary32 RK3 flux/volume arithmetic; see dg_stepper_create_precision)."""
+        binary32 RK3 flux/volume arithmetic; see dg_stepper_create_precision).
+        eps: permittivity per mesh element (dg_stepper_set_permittivity)."""
         self.lib = load_library()
         self.sim = sim
         self.h = self.lib.dg_stepper_create_precision(C.byref(sim.s),
                                                       self.PRECISIONS[precision])
+        if eps is not None:
+            self._eps = np.ascontiguousarray(eps, dtype=np.float64)
+            assert self._eps.shape == (sim.mesh.nelem,)
+            self.lib.dg_stepper_set_permittivity(self.h, dptr(self._eps))
         self.lib.dg_stepper_init(self.h)
 
     def advance(self, k):

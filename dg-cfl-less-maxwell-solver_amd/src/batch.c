@@ -155,6 +155,7 @@ void dg_batch_times(gdn_batch *b, double *t)
 void dg_batch_finish(gdn_batch *b)
 {
 	gdh_ctx *c = b->ctx;
+	gdh_synchronize(c); /* aborts if a sweep's bounded wait expired */
 	if (b->kinetic)
 		gdh_batch_ghosts(c); /* final ghost update at t (src/dg/relaxation.c:175) */
 	const int ne = b->e0[b->count], nb = b->b0[b->count];

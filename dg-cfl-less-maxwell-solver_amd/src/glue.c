@@ -170,6 +170,12 @@ void gdn_model_describe(const gdn_model *mdl, int kinetic, gdh_model_desc *out)
 			out->imposed_par[3 + d] = p->cm_a[d];
 		}
 		out->imposed_par[6] = p->cm_omega;
+	} else if (g == maxwell_imposed_gaussian_pulse) {
+		out->imposed_kind = GDH_IMPOSED_GAUSSIAN_PULSE;
+		for (int d = 0; d < 3; d++)
+			out->imposed_par[d] = p->gp_c[d];
+		out->imposed_par[3] = p->gp_sigma;
+		out->imposed_par[4] = p->gp_amp;
 	} else {
 		out->imposed_kind = GDH_IMPOSED_HOST;
 	}

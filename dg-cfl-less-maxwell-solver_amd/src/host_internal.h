@@ -9,6 +9,7 @@
 struct gdn_imposed_params {
 	double pw_k[3], pw_e[3], pw_h[3], pw_omega, pw_phase;
 	double cm_k[3], cm_a[3], cm_omega;
+	double gp_c[3], gp_sigma, gp_amp;
 };
 const struct gdn_imposed_params *gdn_imposed_params(void);
 

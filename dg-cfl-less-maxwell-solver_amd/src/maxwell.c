@@ -200,6 +200,10 @@ static struct gdn_imposed_params g_par = {
 	.cm_k = { M_PI, M_PI, M_PI },
 	.cm_a = { 1, -1, 0 },
 	.cm_omega = M_PI * 1.7320508075688772,
+	/* Gaussian pulse: centred in the unit cube, sigma 0.08 */
+	.gp_c = { 0.5, 0.5, 0.5 },
+	.gp_sigma = 0.08,
+	.gp_amp = 1.0,
 };
 
 const struct gdn_imposed_params *gdn_imposed_params(void)
@@ -221,6 +225,39 @@ void maxwell_plane_wave_set(const gdn_real k[3], const gdn_real e0[3],
 	g_par.pw_h[2] = (k[0] * e0[1] - k[1] * e0[0]) / kk;
 	g_par.pw_omega = kk;
 	g_par.pw_phase = phase;
+}
+
+/* the same wave in a medium of permittivity eps (mu = 1): speed
+ * 1 / sqrt(eps), H = sqrt(eps) (k/|k|) x e0 s */
+void maxwell_plane_wave_set_eps(const gdn_real k[3], const gdn_real e0[3], gdn_real phase,
+				gdn_real eps)
+{
+	maxwell_plane_wave_set(k, e0, phase);
+	const gdn_real r = sqrt(eps);
+	for (int d = 0; d < 3; d++)
+		g_par.pw_h[d] *= r;
+	g_par.pw_omega /= r;
+}
+
+void maxwell_gaussian_pulse_set(const gdn_real center[3], gdn_real sigma, gdn_real amp)
+{
+	for (int d = 0; d < 3; d++)
+		g_par.gp_c[d] = center[d];
+	g_par.gp_sigma = sigma;
+	g_par.gp_amp = amp;
+}
+
+/* E_z = amp exp(-|x - c|^2 / (2 sigma^2)), every other field 0, at any t:
+ * the initial state of a pulse and the data kept on the boundary */
+void maxwell_imposed_gaussian_pulse(const gdn_real *x, const gdn_real t, gdn_real *w)
+{
+	(void)t;
+	const struct gdn_imposed_params *p = &g_par;
+	const gdn_real d0 = x[0] - p->gp_c[0], d1 = x[1] - p->gp_c[1], d2 = x[2] - p->gp_c[2];
+	const gdn_real r2 = d0 * d0 + d1 * d1 + d2 * d2;
+	for (int q = 0; q < 6; q++)
+		w[q] = 0;
+	w[2] = p->gp_amp * exp(-r2 / (2 * p->gp_sigma * p->gp_sigma));
 }
 
 void maxwell_imposed_plane_wave(const gdn_real *x, const gdn_real t,

@@ -67,6 +67,11 @@ gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision)
 	return st;
 }
 
+void dg_stepper_set_permittivity(gdn_stepper *st, const gdn_real *eps)
+{
+	gdh_set_permittivity(st->ctx, eps);
+}
+
 void dg_stepper_init(gdn_stepper *st)
 {
 	double *wex = NULL;

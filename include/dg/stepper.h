@@ -33,6 +33,9 @@ gdn_stepper *dg_stepper_create(gdn_simulation *simu);
 #define GDN_PRECISION_FP64 0
 #define GDN_PRECISION_FP32 1
 gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision);
+/* Permittivity per mesh element (gdh_set_permittivity; no counterpart in
+ * the reference); before dg_stepper_init. */
+void dg_stepper_set_permittivity(gdn_stepper *st, const gdn_real *eps);
 void dg_stepper_init(gdn_stepper *st);
 void dg_stepper_advance(gdn_stepper *st, int nsteps);
 gdn_real dg_stepper_time(const gdn_stepper *st);

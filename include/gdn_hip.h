@@ -35,6 +35,7 @@ enum gdh_imposed_kind {
 	GDH_IMPOSED_STATIONARY = 3, /* maxwell_imposed_stationary */
 	GDH_IMPOSED_PLANE_WAVE = 4, /* maxwell_imposed_plane_wave */
 	GDH_IMPOSED_CAVITY_MODE = 5, /* maxwell_imposed_cavity_mode */
+	GDH_IMPOSED_GAUSSIAN_PULSE = 6, /* maxwell_imposed_gaussian_pulse */
 };
 
 enum gdh_kinetic_kind {
@@ -152,11 +153,27 @@ void gdh_apply_relaxation(gdh_ctx *ctx);
  * GDH_IMPOSED_HOST, wexact holds the exact solution at the element nodes. */
 double gdh_error_l2(gdh_ctx *ctx, double t, const double *wexact);
 
+/* Permittivity eps > 0 per mesh element (mu = 1), no counterpart in the
+ * reference (which hard-wires eps = mu = 1 in its flux,
+ * src/models/maxwell.c:152-192). RK3: eps dE/dt = curl H, the E rows of each
+ * stage's M^-1 F w scaled by 1/eps of the element; the upwind flux stays the
+ * reference's (consistent and dissipative for any eps). Kinetic scheme: the
+ * state carries the conserved fields (D, B) = (eps E, H); the transport is
+ * unchanged and the equilibrium f_eq(w) takes its physical flux on
+ * (D / eps, B), so that sum_v f_eq = (D, B) and sum_v v f_eq = A(e_k)
+ * (E, H); initial and ghost data are f_eq of (eps E, H) of the imposed
+ * (E, H) (ghosts with the eps of their element), the L2 error compares
+ * D / eps with E. Call before gdh_init_solution; not with the projector
+ * boundary state. */
+void gdh_set_permittivity(gdh_ctx *ctx, const double *eps);
+
 /* Discrete energy 1/2 w^T M w of the device state (M the assembled P2 mass
  * matrix, src/simulation/simulation.c:218-261; w the macro fields, or
  * sum_v f_v for the kinetic scheme), element DoFs only, in binary64 with a
- * fixed reduction order. No counterpart in the reference, which reports no
- * energy; Maxwell conserves it and the upwind scheme dissipates it. */
+ * fixed reduction order; with a permittivity 1/2 (eps |E|^2 + |H|^2)
+ * (RK3) or 1/2 (|D|^2 / eps + |B|^2) (kinetic). No counterpart in the
+ * reference, which reports no energy; Maxwell conserves it and the upwind
+ * scheme dissipates it. */
 double gdh_energy(gdh_ctx *ctx);
 
 /* The same energy of a state held in device memory by the caller, in the
@@ -262,7 +279,7 @@ int gdh_cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const int
  * and boundary faces [bface_first[i], bface_first[i+1]), with its own
  * closed-form exact data (the model's imposed_kind with the parameter block
  * par[GDH_IMPOSED_NPAR * i ...]), time step dt[i] and clock (starting at 0).
- * eps: permittivity per element (RK3 only; eps dE/dt = curl H), or NULL.
+ * eps: permittivity per element (as gdh_set_permittivity), or NULL.
  * Call after gdh_create_ex and before gdh_relax_setup (kinetic) and
  * gdh_init_solution. */
 void gdh_batch_setup(gdh_ctx *ctx, int ninst, const int *elem_first,

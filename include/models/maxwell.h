@@ -52,6 +52,18 @@ void maxwell_plane_wave_set(const gdn_real k[3], const gdn_real e0[3],
 void maxwell_imposed_plane_wave(const gdn_real *x, const gdn_real t,
 				gdn_real *w);
 
+/* The plane wave in a medium of permittivity eps (mu = 1): speed
+ * 1/sqrt(eps), H = sqrt(eps) (k/|k|) x e0 s, s = sin(k.x - |k| t / sqrt(eps)
+ * + phase); the exact solution for a uniform permittivity eps. */
+void maxwell_plane_wave_set_eps(const gdn_real k[3], const gdn_real e0[3], gdn_real phase,
+				gdn_real eps);
+
+/* Gaussian pulse E_z = amp exp(-|x - c|^2 / (2 sigma^2)), every other field
+ * zero, the same at every t (initial state; boundary data that stay at the
+ * pulse's tail). Default: c = (1/2, 1/2, 1/2), sigma = 0.08, amp = 1. */
+void maxwell_gaussian_pulse_set(const gdn_real center[3], gdn_real sigma, gdn_real amp);
+void maxwell_imposed_gaussian_pulse(const gdn_real *x, const gdn_real t, gdn_real *w);
+
 /* Resonant mode (l,m,n) of the perfectly conducting unit cube:
  *   E = (a0 cx sy sz, a1 sx cy sz, a2 sx sy cz) cos(om t),
  *   H = -(sin(om t)/om) curl(E(t=0)),

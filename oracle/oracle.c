@@ -781,6 +781,16 @@ static void exact(const oimposed *ip, const double *x, double t, double *w)
 		w[5] = -st * (k[0] * a[1] - k[1] * a[0]) * c[0] * c[1] * s[2];
 		break;
 	}
+	case 6: { /* Gaussian pulse E_z = amp exp(-|x - c|^2 / (2 sigma^2)) */
+		const double *c = ip->p;
+		double r2 = 0;
+		for (int d = 0; d < 3; d++)
+			r2 += (x[d] - c[d]) * (x[d] - c[d]);
+		for (int q = 0; q < 6; q++)
+			w[q] = 0;
+		w[2] = ip->p[4] * exp(-r2 / (2 * ip->p[3] * ip->p[3]));
+		break;
+	}
 	default:
 		die("unknown exact solution");
 	}
@@ -812,13 +822,15 @@ static void kin_set(okin *k, int kind)
 	}
 }
 
-/* f_eq via the physical flux qM = flux(w, w, e_k) */
-static void kin_feq(const okin *k, const double *w, double *feq)
+/* f_eq via the physical flux qM = flux(wf, wf, e_k) with wf = (D / eps, B)
+ * the fields of the conserved state w = (D, B) (wf = w without permittivity) */
+static void kin_feq_eps(const okin *k, const double *w, double ieps, double *feq)
 {
 	const double e[3][3] = { { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+	const double wf[6] = { w[0] * ieps, w[1] * ieps, w[2] * ieps, w[3], w[4], w[5] };
 	double fl[3][6];
 	for (int d = 0; d < 3; d++)
-		flux_upwind(w, w, e[d], fl[d]);
+		flux_upwind(wf, wf, e[d], fl[d]);
 	if (k->kind == 1) {
 		const double l2 = k->lambda * k->lambda;
 		for (int iw = 0; iw < 6; iw++)
@@ -964,6 +976,11 @@ void orc_fr_projector(int kind, const double *fL, const double *vn, const double
 	for (int c = 0; c < 6 * nm; c++)
 		q[c] += qM[c];
 	kin_from_moments(&k, wR, q, fR);
+}
+
+static void kin_feq(const okin *k, const double *w, double *feq)
+{
+	kin_feq_eps(k, w, 1.0, feq);
 }
 
 /* =================================================================== */
@@ -1331,6 +1348,7 @@ typedef struct {
 	double t_asm, t_run;
 	int nthreads;
 	int fr_projector; /* boundary state: 0 f_eq(wI), 1 the projector form */
+	double *eps; /* permittivity per element (NULL: 1) */
 	/* assembled operators */
 	csr Minv, F; /* RK3 */
 	csr M1[8], M2[8];
@@ -1369,6 +1387,20 @@ void *orc_sim_create(void *mesh, int scheme, int imposed_kind,
 void orc_sim_set_lambda(void *h, double lambda)
 {
 	((osim *)h)->lambda = lambda;
+}
+
+/* permittivity per element (copied; NULL: 1); before solve / operators */
+void orc_sim_set_eps(void *h, const double *eps)
+{
+	osim *s = h;
+	if (s->assembled)
+		die("orc_sim_set_eps after the operators were assembled");
+	free(s->eps);
+	s->eps = NULL;
+	if (eps) {
+		s->eps = xc(s->m->nelem, sizeof(double));
+		memcpy(s->eps, eps, sizeof(double) * s->m->nelem);
+	}
 }
 
 void orc_sim_set_fr_projector(void *h, int on)
@@ -1419,6 +1451,31 @@ static inline size_t kidx(const osim *s, int im, int ipg)
 	return (size_t)im * s->neq + ipg;
 }
 
+/* permittivity of element e and of DoF ipg (ghosts: their L element) */
+static double eps_elem(const osim *s, int e)
+{
+	return s->eps ? s->eps[e] : 1.0;
+}
+
+static double eps_dof(const osim *s, int ipg)
+{
+	const omesh *m = s->m;
+	if (!s->eps)
+		return 1.0;
+	if (ipg < NQN * m->nelem)
+		return s->eps[ipg / NQN];
+	return s->eps[m->f2e[4 * m->bface[(ipg - NQN * m->nelem) / NQV]]];
+}
+
+/* the kinetic state of imposed fields w = (E, H) at DoF ipg: f_eq of the
+ * conserved fields (eps E, H) with the flux of (E, H) */
+static void kin_state(const osim *s, int ipg, const double *w, double *f)
+{
+	const double ep = eps_dof(s, ipg);
+	const double wc[6] = { ep * w[0], ep * w[1], ep * w[2], w[3], w[4], w[5] };
+	kin_feq_eps(&s->kin, wc, 1.0 / ep, f);
+}
+
 /* src/dg/dg.c:32-59 */
 static void init_sol(osim *s)
 {
@@ -1430,7 +1487,7 @@ static void init_sol(osim *s)
 			const double *x = m->xq + 3 * m->e2q[ipg];
 			exact(&s->ip, x, 0., w);
 			if (s->scheme) {
-				kin_feq(&s->kin, w, f);
+				kin_state(s, ipg, w, f);
 				for (int im = 0; im < s->mm; im++)
 					s->wn[kidx(s, im, ipg)] = f[im];
 			} else {
@@ -1463,7 +1520,7 @@ static void update_boundary(osim *s, double t)
 				for (int im = 0; im < s->mm; im++)
 					s->wn[kidx(s, im, ipgR)] = f[im];
 			} else if (s->scheme) {
-				kin_feq(&s->kin, w, f);
+				kin_state(s, ipgR, w, f);
 				for (int im = 0; im < s->mm; im++)
 					s->wn[kidx(s, im, ipgR)] = f[im];
 			} else {
@@ -1485,9 +1542,12 @@ static void asm_inv_mass(osim *s, coo *t)
 		for (int i = 0; i < NQN; i++)
 			for (int j = 0; j < NQN; j++) {
 				double v = T_imass[i][j] / det;
+				/* RK3, eps dE/dt = curl H: the E rows of M^-1 carry
+				 * 1/eps (the kinetic transport has no permittivity) */
 				for (int iw = 0; iw < nw; iw++)
 					coo_add(t, (NQN * e + i) * nw + iw,
-						(NQN * e + j) * nw + iw, v);
+						(NQN * e + j) * nw + iw,
+						!s->scheme && iw < 3 ? v / eps_elem(s, e) : v);
 			}
 	}
 }
@@ -1714,7 +1774,7 @@ static void relax_all(osim *s, int with_ghosts)
 			for (int iv = 0; iv < s->nv; iv++)
 				w[iw] += f[iw * s->nv + iv];
 		}
-		kin_feq(&s->kin, w, feq);
+		kin_feq_eps(&s->kin, w, 1.0 / eps_dof(s, ipg), feq);
 		for (int im = 0; im < s->mm; im++)
 			s->wn[kidx(s, im, ipg)] = om * feq[im] + (1. - om) * f[im];
 	}
@@ -1948,6 +2008,8 @@ double orc_error_l2(void *h)
 					wl[iw] = 0;
 					for (int iv = 0; iv < s->nv; iv++)
 						wl[iw] += s->wn[kidx(s, iw * s->nv + iv, ipg)];
+					if (iw < 3)
+						wl[iw] /= eps_elem(s, e); /* E = D / eps */
 				} else {
 					wl[iw] = s->wn[(size_t)6 * ipg + iw];
 				}
@@ -1987,9 +2049,12 @@ double orc_energy(void *h)
 					w[i] = s->wn[(size_t)6 * ipg + iw];
 				}
 			}
+			/* 1/2 (eps |E|^2 + |H|^2) (RK3), 1/2 (|D|^2 / eps + |B|^2) */
+			const double wt = iw >= 3 ? 1.0
+					  : s->scheme ? 1.0 / eps_elem(s, e) : eps_elem(s, e);
 			for (int i = 0; i < NQN; i++)
 				for (int j = 0; j < NQN; j++)
-					acc += 0.5 * det * w[i] * T_mass[i][j] * w[j];
+					acc += 0.5 * det * wt * w[i] * T_mass[i][j] * w[j];
 		}
 	}
 	return acc;
@@ -2009,5 +2074,6 @@ void orc_sim_free(void *h)
 			blocklu_free(&s->LU[iv]);
 	}
 	free(s->wn);
+	free(s->eps);
 	free(s);
 }

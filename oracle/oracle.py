@@ -19,7 +19,7 @@ dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
 
 # exact-solution kinds (same numbering as include/gdn_hip.h)
-MACRO_COS, CAVITY_TM, STATIONARY, PLANE_WAVE, CAVITY_MODE = 1, 2, 3, 4, 5
+MACRO_COS, CAVITY_TM, STATIONARY, PLANE_WAVE, CAVITY_MODE, GAUSSIAN_PULSE = 1, 2, 3, 4, 5, 6
 D3Q4, D3Q6 = 1, 2
 UPWIND, METAL = 1, 2
 
@@ -58,6 +58,7 @@ def lib():
             "orc_error_l2": (C.c_double, [C.c_void_p]),
             "orc_energy": (C.c_double, [C.c_void_p]),
             "orc_sim_set_fr_projector": (None, [C.c_void_p, C.c_int]),
+            "orc_sim_set_eps": (None, [C.c_void_p, dp]),
             "orc_proj": (None, [dp, dp, C.c_int, dp]),
             "orc_fr_projector": (None, [C.c_int, dp, dp, dp, dp]),
             "orc_sim_free": (None, [C.c_void_p]),
@@ -118,7 +119,7 @@ class OSim:
     """Assembled-matrix solve of the reference algorithm on an OMesh."""
 
     def __init__(self, mesh, scheme, imposed_kind, imposed_par=None, kin=0,
-                 bflux=UPWIND, lam=None, threads=None, fr_projector=False):
+                 bflux=UPWIND, lam=None, threads=None, fr_projector=False, eps=None):
         par = np.zeros(16)
         if imposed_par is not None:
             par[: len(imposed_par)] = imposed_par
@@ -128,6 +129,10 @@ class OSim:
             lib().orc_sim_set_lambda(self.h, lam)
         lib().orc_sim_set_threads(self.h, threads or os.cpu_count() or 1)
         lib().orc_sim_set_fr_projector(self.h, int(fr_projector))
+        if eps is not None:
+            self._eps = np.ascontiguousarray(eps, dtype=np.float64)
+            assert self._eps.shape == (mesh.nelem,)
+            lib().orc_sim_set_eps(self.h, _d(self._eps))
         self.info()
 
     def info(self):

@@ -103,3 +103,31 @@ class HipRuntime:
         out = np.zeros(n, dtype)
         self.check(self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, self.D2H), "hipMemcpy")
         return out
+
+
+def plane_wave_eps_par(k, e0, eps, phase=0.0):
+    """Parameter block of the plane wave in a uniform medium of permittivity
+    eps: speed 1/sqrt(eps), H = sqrt(eps) (k/|k|) x e0 s."""
+    par = plane_wave_par(k, e0, phase)
+    par[6:9] *= np.sqrt(eps)
+    par[9] /= np.sqrt(eps)
+    return par
+
+
+def gaussian_par(center=(0.5, 0.5, 0.5), sigma=0.08, amp=1.0):
+    par = np.zeros(16)
+    par[0:3], par[3], par[4] = center, sigma, amp
+    return par
+
+
+def block_eps(path, nblock=2, seed=3, lo=1.0, hi=4.0):
+    """Piecewise-constant permittivity in [lo, hi] per cell of an nblock^3
+    grid over the unit cube, by element centroid (numpy default_rng(seed))."""
+    om = O.OMesh(path)
+    a = om.arrays()
+    X = a["node"].reshape(-1, 3)[a["elem2qnode"][: 10 * om.nelem]].reshape(om.nelem, 10, 3)
+    c = X[:, :4, :].mean(axis=1)
+    om.free()
+    table = np.random.default_rng(seed).uniform(lo, hi, size=(nblock,) * 3)
+    idx = np.minimum((c * nblock).astype(int), nblock - 1)
+    return table[idx[:, 0], idx[:, 1], idx[:, 2]]
