@@ -37,6 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "dg-cfl-less-maxwell-solver_amd"))
 import numpy as np  # noqa: E402
 
 import gdon3d as G  # noqa: E402
+from workloads import batch_instances, learned_eps  # noqa: E402
 
 METRIC = "DoF-updates/s at fixed L2 error, P2 Maxwell RK3 & relaxation; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
@@ -186,36 +187,11 @@ def copy_calibration(args):
           file=sys.stderr, flush=True)
 
 
-def batch_instances(count, seed=5):
-    """The batched workload: `count` Kuhn cubes with n log-uniform in
-    [4, 24], a plane wave of random direction on the unit sphere with
-    |k| = 2 pi m, m in {1, 2, 3}, E0 random unit vector orthogonal to k, and
-    eps uniform in [1, 4] per 4^3 block of cubes (numpy default_rng(seed))."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        n = int(np.rint(np.exp(rng.uniform(np.log(4.0), np.log(24.0)))))
-        d = rng.normal(size=3)
-        d /= np.linalg.norm(d)
-        m = int(rng.integers(1, 4))
-        e0 = np.cross(d, rng.normal(size=3))
-        e0 /= np.linalg.norm(e0)
-        k = 2 * np.pi * m * d
-        par = np.zeros(16)
-        par[0:3], par[3:6], par[6:9], par[9] = k, e0, np.cross(d, e0), np.linalg.norm(k)
-        nb = (n + 3) // 4
-        table = rng.uniform(1.0, 4.0, size=(nb, nb, nb))
-        cube = np.arange(6 * n ** 3) // 6  # 6 tets per cube, cube i + n (j + n k)
-        i, j, kk = cube % n, (cube // n) % n, cube // (n * n)
-        out.append((n, par, table[i // 4, j // 4, kk // 4]))
-    return out
-
-
 def bench_batch(args):
-    """Batched form: 256 instances (batch_instances), FP32 fields by default,
-    all instances (and velocities) per launch, captured-graph steps,
-    autotuned tiles. --batch-kind rk3 (with eps per block) or relax (D3Q4,
-    cfl 5, eps = 1: the kinetic model has no permittivity)."""
+    """Batched form: 256 instances (workloads.batch_instances), FP32 fields
+    by default, all instances (and velocities) per launch, captured-graph
+    steps, autotuned tiles, eps per 4^3 block of cubes. --batch-kind rk3
+    (CFL 0.5) or relax (D3Q4, cfl 5)."""
     kin = args.batch_kind == "relax"
     inst = batch_instances(args.instances)
     t0 = time.perf_counter()
@@ -231,12 +207,12 @@ def bench_batch(args):
     nelem = sum(6 * n ** 3 for n, _, _ in inst)
     seq = None
     if args.batch_sequential:
-        # the same instances one stepper each (eps = 1), advanced one after
-        # the other: what running them without the batch costs
+        # the same instances one stepper each, advanced one after the
+        # other: what running them without the batch costs
         steppers = []
-        for sm, par in zip(sims, pars):
+        for sm, par, e in zip(sims, pars, eps):
             G.set_plane_wave(par[0:3], par[3:6])
-            steppers.append(G.Stepper(sm, precision=args.precision))
+            steppers.append(G.Stepper(sm, precision=args.precision, eps=e))
         for st in steppers:
             st.advance(2)
         for st in steppers:
@@ -249,8 +225,7 @@ def bench_batch(args):
         seq = dof * args.steps / (time.perf_counter() - ts)
         for st in steppers:
             st.free()
-    b = G.Batch(sims, np.array(pars), eps=None if kin else np.concatenate(eps),
-                precision=args.precision)
+    b = G.Batch(sims, np.array(pars), eps=np.concatenate(eps), precision=args.precision)
     b.sync()
     t_setup = time.perf_counter() - t0
     tiles = b.tiles()
@@ -280,8 +255,8 @@ def bench_batch(args):
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": f"batched form: {args.instances} Kuhn cubes, n log-uniform "
                                f"in [4,24] (seed 5), random plane waves |k|=2pi m, "
-                               + ("D3Q4 relaxation cfl 5, eps=1" if kin else
-                                  "RK3 CFL 0.5, eps in [1,4] per 4^3 block")
+                               + ("D3Q4 relaxation cfl 5" if kin else "RK3 CFL 0.5")
+                               + ", eps in [1,4] per 4^3 block"
                                + f", {args.precision.upper()}, captured-graph steps",
                    "instances": args.instances, "elements": nelem, "dof": dof,
                    "tiles_rk3_relax": list(tiles),
@@ -289,8 +264,7 @@ def bench_batch(args):
                    "l2_error_median": float(np.median(err)),
                    "l2_error_max": float(err.max()),
                    "l2_note": "against the eps=1 plane wave; with eps != 1 blocks "
-                              "this is not the exact solution" if not kin else
-                              "against the exact plane wave"},
+                              "this is not the exact solution"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "whole step (all kernels, graph replay)",

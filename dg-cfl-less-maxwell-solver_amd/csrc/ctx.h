@@ -87,7 +87,9 @@ struct gdh_ctx {
 	double *dinv; /* per item: 10x10 inverse of the diagonal block */
 	uint8_t *cls; /* per (e, f, v): 0 none, 1 outflow, 2 inflow */
 	struct ItemDesc *desc; /* per item: element, velocity, inflow faces */
-	unsigned long long *ll; /* per item: 60 values as epoch-tagged words */
+	unsigned long long *ll; /* per item: 60 hand-off words, one per value in the
+				 * state precision, epoch parity in the lowest bit */
+	int ll_partial; /* the last sweep left some items' words unwritten */
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */

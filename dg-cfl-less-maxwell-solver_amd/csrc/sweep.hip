@@ -176,15 +176,19 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 /* Persistent sweep: hand-off of values between workgroups               */
 /* ------------------------------------------------------------------ */
 /*
- * Each finished value is published as two 64-bit words (epoch << 32 | high
- * half) and (epoch << 32 | low half), written with relaxed agent-scope
- * stores. 64-bit accesses are single-copy atomic, so a reader that sees the
- * current epoch in both words holds the whole value: there is no separate
- * flag, no fence and no wait for the store to complete on the producer
- * side, and the consumer polls the data it needs directly. The words go
- * through the per-XCD L2 to the coherent memory side (agent scope), so the
- * hand-off works across XCDs. The buffer starts zeroed and the epoch of the
- * k-th sweep is k (never 0).
+ * Each published value is one word of its own precision (binary64: 64
+ * bits, binary32: 32 bits) whose lowest mantissa bit carries the parity of
+ * the sweep's epoch (the value is rounded once more by that: at most one
+ * unit in the last place). Words are written and read with relaxed
+ * agent-scope atomics, single-copy atomic and written through the per-XCD
+ * L2 to the coherent memory side, so a reader that sees the current parity
+ * holds the whole value: no separate flag, no fence, no wait for the store
+ * on the producer side, and the consumer polls the data it needs directly.
+ * Every item that publishes does so in every sweep, so a word holds either
+ * the previous sweep's value (the other parity) or this one's. The buffer
+ * starts zeroed (parity 0) and the first epoch is 1; after a sweep in which
+ * some items did not publish (the cluster sweep) and before one that reads
+ * every item, the buffer is zeroed and the epochs restart.
  */
 typedef unsigned long long u64;
 
@@ -518,62 +522,70 @@ __device__ inline u64 ll_load(const u64 *p)
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline void ll_store(u64 *p, u64 x)
+__device__ inline unsigned ll_load(const unsigned *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename W> __device__ inline void ll_store(W *p, W x)
 {
 	__hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Hand-off words of the 60 values of one item (v, e) at ll + 60 words (v ne
- * + e): binary64 as a block of 60 high-half words (epoch << 32 | high
- * half) followed by 60 low-half words (epoch << 32 | low half), so that
- * each store instruction of an item writes whole lines; binary32 as 60
- * words (epoch << 32 | bits). */
+/* Hand-off words of the 60 values of one item (v, e): 60 words W at
+ * ll + 60 (v ne + e) in units of W, value k at word k. */
 template <typename R> struct Handoff;
 template <> struct Handoff<double> {
-	static constexpr int words = 2;
-	__device__ static void put(u64 *item, int k, double x, unsigned epoch)
+	using W = u64;
+	__device__ static W tag(double x, unsigned epoch)
 	{
-		const u64 bits = (u64)__double_as_longlong(x), tag = (u64)epoch << 32;
-		ll_store(item + k, tag | (bits >> 32));
-		ll_store(item + 60 + k, tag | (bits & 0xffffffffull));
+		return ((u64)__double_as_longlong(x) & ~1ull) | (u64)(epoch & 1u);
 	}
-	__device__ static void load(const u64 *item, int k, u64 (&w)[2])
+	__device__ static void put(W *item, int k, double x, unsigned epoch)
 	{
-		w[0] = ll_load(item + k);
-		w[1] = ll_load(item + 60 + k);
+		ll_store(item + k, tag(x, epoch));
 	}
-	/* true (and x set) when both words carry the epoch */
-	__device__ static bool decode(const u64 (&w)[2], unsigned epoch, double &x)
+	__device__ static bool fresh(W w, unsigned epoch)
 	{
-		if ((unsigned)(w[0] >> 32) != epoch || (unsigned)(w[1] >> 32) != epoch)
-			return false;
-		x = __longlong_as_double((long long)((w[0] << 32) | (w[1] & 0xffffffffull)));
-		return true;
+		return (unsigned)(w & 1ull) == (epoch & 1u);
+	}
+	__device__ static double value(W w)
+	{
+		return __longlong_as_double((long long)w);
 	}
 };
 template <> struct Handoff<float> {
-	static constexpr int words = 1;
-	__device__ static void put(u64 *item, int k, float x, unsigned epoch)
+	using W = unsigned;
+	__device__ static W tag(float x, unsigned epoch)
 	{
-		ll_store(item + k, ((u64)epoch << 32) | (u64)__float_as_uint(x));
+		return (__float_as_uint(x) & ~1u) | (epoch & 1u);
 	}
-	__device__ static void load(const u64 *item, int k, u64 (&w)[2])
+	__device__ static void put(W *item, int k, float x, unsigned epoch)
 	{
-		w[0] = ll_load(item + k);
+		ll_store(item + k, tag(x, epoch));
 	}
-	__device__ static bool decode(const u64 (&w)[2], unsigned epoch, double &x)
+	__device__ static bool fresh(W w, unsigned epoch)
 	{
-		if ((unsigned)(w[0] >> 32) != epoch)
-			return false;
-		x = (double)__uint_as_float((unsigned)(w[0] & 0xffffffffull));
-		return true;
+		return (w & 1u) == (epoch & 1u);
+	}
+	__device__ static double value(W w)
+	{
+		return (double)__uint_as_float(w);
 	}
 };
 
 /* hand-off words of item (v, e) */
-template <typename R> __device__ inline u64 *ll_item(u64 *ll, int v, int e, int ne)
+template <typename R>
+__device__ inline typename Handoff<R>::W *ll_item(u64 *ll, int v, int e, int ne)
 {
-	return ll + Handoff<R>::words * 60 * ((long)v * ne + e);
+	return reinterpret_cast<typename Handoff<R>::W *>(ll) + 60 * ((long)v * ne + e);
+}
+
+/* the value as a consumer of the hand-off reads it (tag bit included) */
+template <typename R> __device__ inline R published(R x, unsigned epoch)
+{
+	const typename Handoff<R>::W w = Handoff<R>::tag(x, epoch);
+	return (R)Handoff<R>::value(w);
 }
 
 /*
@@ -588,8 +600,8 @@ __device__ inline bool ll_sentinels(const ItemBuf &B, u64 *ll, int ne, unsigned 
 	const int t = threadIdx.x;
 	bool wait = false;
 	if (t < B.nslot)
-		wait = (unsigned)(ll_load(ll_item<R>(ll, B.v, B.nb[t], ne) + (B.map[t] & 15)) >>
-				  32) != epoch;
+		wait = !Handoff<R>::fresh(ll_load(ll_item<R>(ll, B.v, B.nb[t], ne) + (B.map[t] & 15)),
+					  epoch);
 	return __ballot(wait) == 0;
 }
 
@@ -604,20 +616,19 @@ __device__ inline unsigned ll_poll(const ItemBuf &B, u64 *ll, int ne, unsigned e
 {
 	const int t = lane_id();
 	const int q = t / 6, b = t % 6;
-	u64 w[3][2];
+	typename Handoff<R>::W w[3];
 	const int v = uni(B.v);
 #pragma unroll
 	for (int s = 0; s < 3; s++) {
-		w[s][0] = w[s][1] = 0;
+		w[s] = 0;
 		if (pending & (1u << s))
-			Handoff<R>::load(ll_item<R>(ll, v, uni(B.nb[s]), ne),
-					 q * 10 + ((uni(B.map[s]) >> (4 * b)) & 15), w[s]);
+			w[s] = ll_load(ll_item<R>(ll, v, uni(B.nb[s]), ne) + q * 10 +
+				       ((uni(B.map[s]) >> (4 * b)) & 15));
 	}
 #pragma unroll
 	for (int s = 0; s < 3; s++) {
-		double x;
-		if ((pending & (1u << s)) && Handoff<R>::decode(w[s], epoch, x)) {
-			X[s][t] = x;
+		if ((pending & (1u << s)) && Handoff<R>::fresh(w[s], epoch)) {
+			X[s][t] = Handoff<R>::value(w[s]);
 			pending &= ~(1u << s);
 		}
 	}
@@ -662,7 +673,7 @@ __device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], 
 		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 		if (wl)
-			wl[t] = (double)wr;
+			wl[t] = (double)published(wr, epoch);
 	}
 	__syncthreads();
 }
@@ -901,10 +912,10 @@ __device__ inline unsigned cl_poll(int hz, int hw, int da, int db, unsigned um, 
 {
 	const int t = lane_id();
 	const int q = t / 6, b = t % 6;
-	u64 w[CL_XMAX][2];
+	typename Handoff<R>::W w[CL_XMAX];
 #pragma unroll
 	for (int x = 0; x < CL_XMAX; x++) {
-		w[x][0] = w[x][1] = 0;
+		w[x] = 0;
 		if (!(um & (1u << x)))
 			continue;
 		const int jf = uni(((x < 4 ? hz : hw) >> (8 * (x & 3))) & 255);
@@ -912,15 +923,13 @@ __device__ inline unsigned cl_poll(int hz, int hw, int da, int db, unsigned um, 
 		const int nb = cl_dword(da, db, 20 * j + 12 + f);
 		const int map = cl_dword(da, db, 20 * j + 16 + f);
 		if (t < 36 && (pend & (1u << x)))
-			Handoff<R>::load(ll_item<R>(ll, v, nb, ne), q * 10 + ((map >> (4 * b)) & 15),
-					 w[x]);
+			w[x] = ll_load(ll_item<R>(ll, v, nb, ne) + q * 10 + ((map >> (4 * b)) & 15));
 	}
 #pragma unroll
 	for (int x = 0; x < CL_XMAX; x++) {
-		double val;
 		if ((um & (1u << x)) && t < 36 && (pend & (1u << x)) &&
-		    Handoff<R>::decode(w[x], epoch, val)) {
-			Xc[x][t] = val;
+		    Handoff<R>::fresh(w[x], epoch)) {
+			Xc[x][t] = Handoff<R>::value(w[x]);
 			pend &= ~(1u << x);
 		}
 	}
@@ -1036,7 +1045,8 @@ __device__ inline void cl_item(const DescLanes &D, const PrepLoads<R, 1> &P,
 		__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
 		if (flags & (1 << 28))
 			Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
-		wl[t] = (double)wr;
+		/* in-cluster consumers take the value as the hand-off carries it */
+		wl[t] = (double)published(wr, epoch);
 	}
 	__syncthreads();
 }
@@ -1299,9 +1309,11 @@ void sweep_setup(gdh_ctx *c)
 		c->items, c->nitems, c->geom, c->nbr, c->nbmap, c->cls, c->kp, c->dt,
 		c->desc, c->bt);
 	HIPCHECK(hipGetLastError());
-	const size_t llb = sizeof(u64) * (c->fp32 ? 1 : 2) * 60 * (size_t)c->nitems;
+	/* one word of the state precision per value */
+	const size_t llb = c->rsz * 60 * (size_t)c->nitems;
 	c->ll = (u64 *)dmalloc(llb);
 	HIPCHECK(hipMemsetAsync(c->ll, 0, llb, c->stream));
+	c->ll_partial = 0;
 	c->status = (int *)dmalloc(sizeof(int));
 	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1332,25 +1344,46 @@ static int grid_cap(int g)
 	return cap > 0 ? std::min(g, cap) : g;
 }
 
+/* zero the hand-off words and restart the epochs at 1 (set-up only: not
+ * in a captured stream) */
+static void ll_restart(gdh_ctx *c)
+{
+	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+	HIPCHECK(hipStreamIsCapturing(c->stream, &cs));
+	if (cs != hipStreamCaptureStatusNone) {
+		fprintf(stderr, "gdon3d-hip: the sweep form changed inside a captured graph\n");
+		abort();
+	}
+	HIPCHECK(hipMemsetAsync(c->ll, 0, c->rsz * 60 * (size_t)c->nitems, c->stream));
+	if (c->epoch_dev)
+		HIPCHECK(hipMemsetAsync(c->epoch_dev, 0, sizeof(unsigned), c->stream));
+	c->epoch = 0;
+	c->ll_partial = 0;
+}
+
 __global__ void k_epoch_bump(unsigned *e)
 {
-	*e = *e + 1u == 0u ? 1u : *e + 1u;
+	const unsigned n = *e + 1u;
+	*e = n == 0u ? 2u : n; /* wrapped: keep the parity alternating */
 }
 
 template <typename R>
 static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 {
+	/* a sweep that reads every item after one in which some items did not
+	 * publish (cluster sweep): zero the words and restart the epochs, so
+	 * that no word left from two sweeps back passes for this one */
+	if (c->ll_partial && c->sweep_mode != 2)
+		ll_restart(c);
 	/* the sweep's epoch: on the host, or (batches, captured in graphs) in
-	 * device memory advanced by a one-thread node */
-	if (c->epoch_dev) {
+	 * device memory advanced by a one-thread node; only its parity reaches
+	 * the words */
+	if (c->epoch_dev)
 		k_epoch_bump<<<1, 1, 0, c->stream>>>(c->epoch_dev);
-	} else if (++c->epoch == 0) {
-		/* 2^32 sweeps: restart the epochs from a zeroed buffer */
-		HIPCHECK(hipMemsetAsync(c->ll, 0,
-					sizeof(u64) * Handoff<R>::words * 60 * (size_t)c->nitems,
-					c->stream));
-		c->epoch = 1;
-	}
+	else if (++c->epoch == 0)
+		c->epoch = 2; /* wrapped: 2^32 is even, keep the alternation */
+	if (c->sweep_mode == 2)
+		c->ll_partial = 1;
 	const ItemDesc *desc = c->desc;
 	int nitems = c->nitems, nv = c->kp.nv, ne = c->ne, neq = c->neq;
 	const double *dinv = c->dinv;
@@ -1919,6 +1952,10 @@ static void relax_choose(gdh_ctx *c)
 	}
 	c->sweep_mode = cand[best][0];
 	c->relax_fused = cand[best][1];
+	/* the words are made consistent for the chosen form now, before any
+	 * step can be captured */
+	if (c->ll_partial && c->sweep_mode != 2)
+		ll_restart(c);
 }
 
 extern "C" int gdh_relax_fused(gdh_ctx *c)
