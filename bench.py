@@ -274,24 +274,40 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def sweep_ms(st, steps):
+    """mean device time of the transport sweep per step over `steps` steps
+    (HIP events on the solver stream)"""
+    st.profile(True)
+    st.advance(steps)
+    prof = st.profile(False)
+    return prof[0] / steps * 1e3
+
+
 def bench_learned(args):
-    """Learned-preconditioner form: relaxation (D3Q4, FP64) on Kuhn 48^3 x 6
-    with +-20% jitter (seed 23) at 20x the explicit CFL (cfl 10). The block
-    network is trained on the GPU on 10^5 items of this mesh (features and
-    exact block inverses), then Richardson block solves (6 random sides per
-    item) on 10^5 other items run to |r - Du| <= 1e-8 |r| with exact
-    fallback: iterations, fallback rate, L2 to the exact solution. The
-    relaxation steps themselves use the exact sweep."""
+    """Learned-preconditioner form: relaxation (D3Q4) on Kuhn 48^3 x 6 with
+    +-20% jitter (seed 23), eps uniform in [1, 4] per element (seed 23), a
+    Gaussian pulse E_z (sigma 0.08, centred), cfl 10 = 20x the explicit CFL.
+    The block network is trained on the GPU on 10^5 items of this mesh
+    (features and exact block inverses). Reported: Richardson block solves
+    (6 random sides per item on 10^5 other items, to |r - Du| <= 1e-8 |r|,
+    exact fallback) with the network and with the identity; then the
+    relaxation steps twice from the same initial state, with the stored
+    exact block inverses and with the network's inverses refined in the
+    sweep (gdh_learned_enable): DoF-updates/s of both, the items on the
+    stored inverse (fallbacks) and the relative L2 difference of the two
+    solutions. value = the learned form's rate."""
     n = args.n if args.n != 32 else 48
-    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    G.set_gaussian_pulse((0.5, 0.5, 0.5), 0.08, 1.0)
     mesh = G.Mesh(kuhn=(n, 0.2, 23))
     nelem = mesh.nelem
-    sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic="d3q4"))
+    eps = learned_eps(nelem, 23)
+    sim = G.Simulation(mesh, G.Model("gaussian_pulse", kinetic="d3q4"))
     sim.set_time(1.0, 0.0, 10.0)
     t0 = time.perf_counter()
-    st = G.Stepper(sim, precision=args.precision)
+    st = G.Stepper(sim, precision=args.precision, eps=eps)
     st.sync()
     t_setup = time.perf_counter() - t0
+    lagged = st.transport_stats()[0]
     nitems = 4 * nelem
     rng = np.random.default_rng(23)
     perm = rng.permutation(nitems)
@@ -304,44 +320,67 @@ def bench_learned(args):
     t_train = time.perf_counter() - t0
     stats = G.learned_solve_stats(st, net, test_idx)
     ident = G.learned_solve_stats(st, None, test_idx)
-    lagged = st.transport_stats()[0]
-    st.advance(args.warmup)
-    st.sync()
-    t0 = time.perf_counter()
-    st.advance(args.steps)
-    st.sync()
-    elapsed = time.perf_counter() - t0
-    err = st.error_l2()
+    dof = 24 * 10 * nelem
+    runs = {}
+    for mode in ("exact", "learned"):
+        lstats = G.learned_enable(st, net if mode == "learned" else None)
+        form = (G.sweep_info(st)["form"], G.relax_fused(st))
+        G.load_library().dg_stepper_init(st.h)
+        st.advance(args.warmup)
+        st.sync()
+        t0 = time.perf_counter()
+        st.advance(args.steps)
+        st.sync()
+        elapsed = time.perf_counter() - t0
+        wn = st.download()
+        ms = sweep_ms(st, args.profile_steps)
+        runs[mode] = dict(value=dof * args.steps / elapsed, elapsed=elapsed, wn=wn,
+                          sweep_ms=ms, stats=lstats, form=form)
+    diff = float(np.linalg.norm(runs["learned"]["wn"] - runs["exact"]["wn"]) /
+                 np.linalg.norm(runs["exact"]["wn"]))
+    t_end = st.t
+    energy = st.energy()
     net.free()
     st.free()
     sim.free()
-    dof = 24 * 10 * nelem
+    lr, ex = runs["learned"], runs["exact"]
     print(f"[bench learned] n={n} {nelem} tets, setup {t_setup:.1f}s, upwind cycles: "
           f"{lagged} lagged faces; training {args.train_steps} steps {t_train:.2f}s loss "
-          f"{hist[:5].mean():.3e} -> {hist[-20:].mean():.3e}; block solves: mean "
+          f"{hist[:5].mean():.3e} -> {hist[-20:].mean():.3e}; Richardson block solves: mean "
           f"{stats[0]:.2f} it, max {stats[1]:.0f}, fallback {stats[2]:.4f}, max rel L2 "
-          f"{stats[3]:.2e} (identity preconditioner: fallback {ident[2]:.3f})",
+          f"{stats[3]:.2e} (identity preconditioner: fallback {ident[2]:.3f}); sweep "
+          f"exact {ex['sweep_ms']:.3f} ms ({ex['form']}), learned {lr['sweep_ms']:.3f} ms "
+          f"({lr['form']}), items on the network / stored inverse {lr['stats'][:2]}, "
+          f"max deviation {lr['stats'][2]:.2e}; learned vs exact solution rel L2 {diff:.2e}",
           file=sys.stderr, flush=True)
-    value = dof * args.steps / elapsed
     line = {
-        "metric": METRIC, "value": value, "unit": "DoF-updates/s", "n_gpus": 1,
+        "metric": METRIC, "value": lr["value"], "unit": "DoF-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "ms_per_step": lr["elapsed"] / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": f"learned-preconditioner form: D3Q4 relaxation, Kuhn {n}^3x6 "
-                               f"= {nelem} tets, jitter 20% seed 23, cfl 10 (20x explicit); "
-                               f"block network trained on 1e5 items, Richardson to 1e-8 "
-                               f"with exact fallback on 1e5 other items",
+                               f"= {nelem} tets, jitter 20% seed 23, eps in [1,4] per element "
+                               f"(seed 23), Gaussian pulse E_z sigma 0.08, cfl 10 (20x "
+                               f"explicit); block inverses from the network refined in the "
+                               f"sweep, stored exact inverse as fallback",
                    "elements": nelem, "lagged_cycle_faces": lagged,
                    "train_steps": args.train_steps, "train_s": t_train,
                    "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
-                   "block_solve_mean_iterations": float(stats[0]),
-                   "block_solve_max_iterations": float(stats[1]),
-                   "block_solve_fallback_rate": float(stats[2]),
-                   "block_solve_max_rel_l2_vs_exact": float(stats[3]),
+                   "richardson_mean_iterations": float(stats[0]),
+                   "richardson_max_iterations": float(stats[1]),
+                   "richardson_fallback_rate": float(stats[2]),
+                   "richardson_max_rel_l2_vs_exact": float(stats[3]),
                    "identity_preconditioner_fallback_rate": float(ident[2]),
-                   "l2_error": err},
+                   "items_on_network_inverse": lr["stats"][0],
+                   "items_on_stored_inverse": lr["stats"][1],
+                   "refined_inverse_max_rel_deviation": lr["stats"][2],
+                   "exact_inverse_dof_updates_per_s": ex["value"],
+                   "gain_over_exact_inverse": lr["value"] / ex["value"],
+                   "sweep_ms_exact": ex["sweep_ms"], "sweep_ms_learned": lr["sweep_ms"],
+                   "sweep_form_exact": list(ex["form"]), "sweep_form_learned": list(lr["form"]),
+                   "learned_vs_exact_solution_rel_l2": diff,
+                   "t_final": t_end, "energy_final": energy},
         "roofline": None,
     }
     print(json.dumps(line), flush=True)
@@ -369,6 +408,9 @@ def main():
     ap.add_argument("--instances", type=int, default=256, help="batch: instances")
     ap.add_argument("--train-steps", type=int, default=2000, help="learned: Adam steps")
     ap.add_argument("--batch-kind", choices=["rk3", "relax"], default="rk3")
+    ap.add_argument("--learned-inverses", action="store_true",
+                    help="relax: the sweep's block inverses from a network trained on 1e5 "
+                         "items of the mesh, refined in the sweep (gdh_learned_enable)")
     ap.add_argument("--batch-sequential", action="store_true",
                     help="batch: also time the instances one stepper each")
     ap.add_argument("--scheme", choices=["relax", "rk3", "copy", "batch", "learned"],
@@ -418,12 +460,22 @@ def main():
     st = G.Stepper(sim, precision=args.precision)
     st.sync()
     t_setup = time.perf_counter() - t0
+    learned = None
+    if args.learned_inverses:
+        idx = np.random.default_rng(23).permutation(4 * nelem)[:100000]
+        x, y = G.learned_data(st, idx)
+        net = G.BlockNet(seed=5)
+        net.set_normalization(G.normalization(x, y))
+        net.train(x, y, steps=args.train_steps, batch=1024, lr=2e-3)
+        learned = G.learned_enable(st, net)
+        net.free()
+        G.load_library().dg_stepper_init(st.h)
     relax_fused = G.relax_fused(st)
     sweep = G.sweep_info(st)
     traffic = SWEEP_TRAFFIC_BYTES_PER_ELEM.get((sweep["form"], relax_fused))
     sim_itermax = sim.s.itermax
     default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
-                        and args.precision == "fp64")
+                        and args.precision == "fp64" and not args.learned_inverses)
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
           f"levels={st.levels} sweep={sweep} "
           f"relax_fused={relax_fused} setup={t_setup:.2f}s",
@@ -507,6 +559,9 @@ def main():
             "sweep_form": sweep["form"],
             "sweep_clusters": sweep["clusters"],
             "sweep_resident_workgroups": sweep["grid"],
+            "learned_inverses": (None if learned is None else
+                                 {"items_on_network": learned[0], "items_on_stored": learned[1],
+                                  "max_rel_deviation": learned[2]}),
             "energy_history": ({"t": hist_t, "energy": hist_e,
                                 "ratio_end": hist_e[-1] / hist_e[0],
                                 "note": "1/2 w^T M w, binary64 device reduction, second "

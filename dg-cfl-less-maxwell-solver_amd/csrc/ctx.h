@@ -61,6 +61,8 @@ struct gdh_ctx {
 	int2 *bface; /* kinetic: per boundary face, its element and local face */
 	double *ieps; /* 1/eps per element (kinetic: mesh order; RK3: by storage
 		       * column), or null */
+	double *lb_x; /* learned block inverses: 8 numbers per sweep item */
+	unsigned *lb_p; /* and the bfloat16 predictions, 50 words per item */
 	double *xnode, *xghost;
 	/* RK3: w = structure-of-arrays interior state W[q][i][e], gh = ghosts
 	 * GH[q][6 bf + a]; kinetic: w = item-major state (kin_at), ghosts
@@ -277,5 +279,6 @@ void relax_launch(gdh_ctx *c, void *f, long n);
 void block_inverse_launch(gdh_ctx *c);
 void set_permittivity(gdh_ctx *c, const double *eps);
 void check_sweep_status(gdh_ctx *c);
+void sweep_set_learned(gdh_ctx *c, const int *ok);
 
 #endif

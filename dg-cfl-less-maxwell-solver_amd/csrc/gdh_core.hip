@@ -164,6 +164,8 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->stage = nullptr;
 	c->bface = nullptr;
 	c->ieps = nullptr;
+	c->lb_x = nullptr;
+	c->lb_p = nullptr;
 	if (scheme) {
 		/* the sweep reads neighbours in mesh order */
 		HIPCHECK(hipMemcpy(c->nbr, mesh->nbr, sizeof(int4) * ne,
@@ -305,7 +307,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->cap_lu, c->cap_t, c->cap_luoff, c->epoch_dev, c->chead, c->cdesc,
 			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
 			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
-			 c->stage, c->bface, c->ieps };
+			 c->stage, c->bface, c->ieps, c->lb_x, c->lb_p };
 	for (void *p : ptrs)
 		if (p)
 			HIPCHECK(hipFree(p));

@@ -20,6 +20,7 @@
  * fixed-order sum and Adam in one kernel, fp32 master weights.
  */
 #include "ctx.h"
+#include "learned_block.h"
 
 using namespace gdh;
 
@@ -808,4 +809,184 @@ extern "C" void gdh_learned_solve_stats(gdh_ctx *c, gdh_mlp *mlp, int n, const i
 	for (void *q : { (void *)di, (void *)pred, (void *)dit, (void *)der })
 		if (q)
 			HIPCHECK(hipFree(q));
+}
+
+/* ------------------------------------------------------------------ */
+/* The network's block inverses inside the transport sweep               */
+/* ------------------------------------------------------------------ */
+
+/* Per sweep item: its block's numbers in binary64 (x[8 it + q]: tau a_k,
+ * k < 3, then tau s_f on the outflow faces, 0 elsewhere; x[8 it + 7] = 0)
+ * and the network's inputs in binary32 (xf[7 it + q]) */
+__global__ void k_item_numbers(const int2 *__restrict__ items, int nitems,
+			       const double *__restrict__ geom, const uint8_t *__restrict__ cls,
+			       KinPar kp, double dt_, BatchTab bt, double *__restrict__ x,
+			       float *__restrict__ xf)
+{
+	const int it = blockIdx.x * blockDim.x + threadIdx.x;
+	if (it >= nitems)
+		return;
+	const int v = items[it].x, e = items[it].y;
+	const double dt = bt.ninst ? bt.dt[bt.einst[e]] : dt_;
+	double ct[9];
+	for (int k = 0; k < 9; k++)
+		ct[k] = geom[10L * e + k];
+	const double tau = THETA_L * dt / geom[10L * e + 9];
+	const double *vv = kp.vi + 3 * v;
+	double q[8];
+	for (int k = 0; k < 3; k++)
+		q[k] = tau * (ct[0 + k] * vv[0] + ct[3 + k] * vv[1] + ct[6 + k] * vv[2]);
+	for (int f = 0; f < 4; f++)
+		q[3 + f] = cls[(4L * e + f) * kp.nv + v] == 1 ? tau * flux_coef(ct, f, vv) : 0.0;
+	q[7] = 0.0;
+	for (int k = 0; k < 8; k++)
+		x[8L * it + k] = q[k];
+	for (int k = 0; k < 7; k++)
+		xf[7L * it + k] = (float)q[k];
+}
+
+/* the predictions (D^-1 - I, binary32) rounded to bfloat16 and packed two
+ * per word: p[50 it + m] = entries 2 m (low half), 2 m + 1 (high half) */
+__global__ void k_pack_pred(const float *__restrict__ pred, long n, unsigned *__restrict__ p)
+{
+	const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 50 * n)
+		return;
+	const unsigned lo = (unsigned short)to_bf16(pred[2 * k]);
+	const unsigned hi = (unsigned short)to_bf16(pred[2 * k + 1]);
+	p[k] = lo | (hi << 16);
+}
+
+/*
+ * The sweep's refinement of each item, one thread per item and in the same
+ * order of operations as the sweep's lanes (learned_inverse in sweep.hip):
+ * steps[it] = S, the fewest steps (at least 2) after which the last
+ * residual R = I - D X before the final update is at most tol (the result
+ * then lies within rounding of the inverse), or 0 when LB_MAX_STEPS do not
+ * reach it (the item keeps the stored exact inverse). dev[it] = max |X_S -
+ * D^-1| / max |D^-1| against the stored exact inverse, res0[it] =
+ * max |I - D X0| of the prediction.
+ */
+__global__ void k_learned_check(int nitems, const double *__restrict__ x,
+				const unsigned *__restrict__ p, const double *__restrict__ dinv,
+				double tol, int *__restrict__ steps, double *__restrict__ dev,
+				double *__restrict__ res0)
+{
+	const int it = blockIdx.x * blockDim.x + threadIdx.x;
+	if (it >= nitems)
+		return;
+	double xs[8], X[100], Dm[100], Rm[100];
+	for (int k = 0; k < 8; k++)
+		xs[k] = x[8L * it + k];
+	for (int m = 0; m < 50; m++) {
+		const unsigned w = p[50L * it + m];
+		const int a = 2 * m, b = 2 * m + 1;
+		X[a] = lb_pred((unsigned short)(w & 0xffffu)) + (a / 10 == a % 10 ? 1.0 : 0.0);
+		X[b] = lb_pred((unsigned short)(w >> 16)) + (b / 10 == b % 10 ? 1.0 : 0.0);
+	}
+	for (int e = 0; e < 100; e++)
+		Dm[e] = lb_entry(xs, e / 10, e % 10);
+	int S = 0;
+	for (int step = 0; step < LB_MAX_STEPS && !S; step++) {
+		double res = 0.0;
+		for (int e = 0; e < 100; e++) {
+			double r = e / 10 == e % 10 ? 1.0 : 0.0;
+			for (int l = 0; l < 10; l++)
+				r -= Dm[10 * (e / 10) + l] * X[10 * l + e % 10];
+			Rm[e] = r;
+			res = fmax(res, fabs(r));
+		}
+		if (step == 0)
+			res0[it] = res;
+		double Xn[100];
+		for (int e = 0; e < 100; e++) {
+			double s = X[e];
+			for (int l = 0; l < 10; l++)
+				s += X[10 * (e / 10) + l] * Rm[10 * l + e % 10];
+			Xn[e] = s;
+		}
+		for (int e = 0; e < 100; e++)
+			X[e] = Xn[e];
+		if (step >= 1 && res <= tol) /* NaN fails */
+			S = step + 1;
+	}
+	double d = 0.0, m = 0.0;
+	for (int e = 0; e < 100; e++) {
+		d = fmax(d, fabs(X[e] - dinv[100L * it + e]));
+		m = fmax(m, fabs(dinv[100L * it + e]));
+	}
+	steps[it] = S;
+	dev[it] = d / m;
+}
+
+/*
+ * Learned block inverses in the transport sweep (no counterpart in the
+ * reference, whose KLU factors each block exactly): the network predicts
+ * every item's D^-1 - I from its seven numbers, stored in bfloat16 (200 B
+ * per item instead of the 800 B of the stored binary64 inverse); the sweep
+ * rebuilds D from the numbers and refines the prediction by Newton-Schulz
+ * steps to the binary64 inverse. Items whose refinement would not converge
+ * (residual above tol before the last step) keep the stored exact inverse.
+ * mlp null: every item back on the stored inverse.
+ */
+extern "C" void gdh_learned_enable(gdh_ctx *c, gdh_mlp *mlp, double tol, double *out)
+{
+	if (!c->scheme || !c->nitems) {
+		fprintf(stderr, "gdon3d-hip: learned block inverses need a set-up kinetic context\n");
+		abort();
+	}
+	const int n = c->nitems;
+	out[0] = out[1] = out[2] = out[3] = 0.0;
+	if (!mlp) {
+		c->kp.lb_p = nullptr;
+		c->kp.lb_x = nullptr;
+		sweep_set_learned(c, nullptr);
+		return;
+	}
+	if (!c->lb_x) {
+		c->lb_x = (double *)dmalloc(sizeof(double) * 8 * (size_t)n);
+		c->lb_p = (unsigned *)dmalloc(sizeof(unsigned) * 50 * (size_t)n);
+	}
+	float *xf = (float *)dmalloc(sizeof(float) * 7 * (size_t)n);
+	float *pred = (float *)dmalloc(sizeof(float) * 100 * (size_t)n);
+	int *ok = (int *)dmalloc(sizeof(int) * (size_t)n);
+	double *dev = (double *)dmalloc(sizeof(double) * (size_t)n);
+	double *r0 = (double *)dmalloc(sizeof(double) * (size_t)n);
+	k_item_numbers<<<nblocks(n, 64), 64, 0, c->stream>>>(c->items, n, c->geom, c->cls, c->kp,
+							      c->dt, c->bt, c->lb_x, xf);
+	HIPCHECK(hipGetLastError());
+	k_mlp_predict<<<((n + 15) / 16 + 3) / 4, 256, 0, c->stream>>>(mlp->w, mlp->wb, mlp->norm,
+								       xf, n, pred);
+	HIPCHECK(hipGetLastError());
+	k_pack_pred<<<nblocks(50L * n, 256), 256, 0, c->stream>>>(pred, n, c->lb_p);
+	HIPCHECK(hipGetLastError());
+	k_learned_check<<<nblocks(n, 64), 64, 0, c->stream>>>(n, c->lb_x, c->lb_p, c->dinv, tol,
+							       ok, dev, r0);
+	HIPCHECK(hipGetLastError());
+	std::vector<int> hok(n);
+	std::vector<double> hdev(n), hr0(n);
+	HIPCHECK(hipMemcpyAsync(hok.data(), ok, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipMemcpyAsync(hdev.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost,
+				c->stream));
+	HIPCHECK(hipMemcpyAsync(hr0.data(), r0, sizeof(double) * n, hipMemcpyDeviceToHost,
+				c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	long good = 0;
+	double dmax = 0.0;
+	for (int k = 0; k < n; k++)
+		if (hok[k]) {
+			good++;
+			dmax = std::max(dmax, hdev[k]);
+		}
+	c->kp.lb_p = c->lb_p;
+	c->kp.lb_x = c->lb_x;
+	sweep_set_learned(c, ok);
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	for (void *q : { (void *)xf, (void *)pred, (void *)ok, (void *)dev, (void *)r0 })
+		HIPCHECK(hipFree(q));
+	std::nth_element(hr0.begin(), hr0.begin() + n / 2, hr0.end());
+	out[0] = (double)good;
+	out[1] = (double)(n - good);
+	out[2] = dmax;
+	out[3] = hr0[n / 2];
 }

@@ -236,6 +236,11 @@ struct KinPar {
 	 * the permittivity of their element). */
 	const double *ieps;
 	const int2 *bface;
+	/* learned block inverses (null: the stored exact ones everywhere):
+	 * per sweep item 50 words of two bfloat16 each, (X0 - I) row-major, and
+	 * the block's 8 numbers in binary64 (7 used) */
+	const unsigned *lb_p;
+	const double *lb_x;
 };
 
 /* 1/eps at DoF ipg (element DoFs, or ghost DoFs through their element) */

@@ -4,6 +4,7 @@
  * the node-local relaxation.
  */
 #include "ctx.h"
+#include "learned_block.h"
 
 using namespace gdh;
 
@@ -196,6 +197,9 @@ struct ItemDesc {
 	double cf[4]; /* s_f dt / det on the inflow faces, 0 elsewhere */
 	int e, v;
 	int flags; /* bit f: inflow face f; bit 4 + f: inflow from the ghosts;
+		    * bit 29: the block inverse is refined in the sweep from the
+		    * network's prediction (KinPar lb_p, lb_x), else it is the
+		    * stored exact one; bits 30-31: its refinement steps - 2;
 		    * bit 8 + f: lagged inflow (cycle broken; value given in xl);
 		    * cluster order only, bits 12 + 4 f .. 15 + 4 f: where the
 		    * trace of face f comes from, inflow face x (0..7) of the
@@ -345,8 +349,11 @@ struct PrepScratch {
  * only) are read by prep_compute itself. */
 template <typename R, int NV> struct PrepLoads {
 	R fr[NV];
-	double d0, d1;
+	double d0, d1; /* exact D^-1 entries t, 64 + t; learned: d0 = number t < 8 */
+	unsigned pw; /* learned: prediction entries 2 t, 2 t + 1 (t < 50) */
 };
+
+#define LB_FLAG (1 << 29)
 
 /*
  * Transport of item (v, e), split around the dependency wait. With
@@ -409,10 +416,10 @@ __device__ inline double uni(double x)
 	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <typename R, int NV>
+template <typename R, int NV, bool LB = false>
 __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict__ fo,
 				  const double *__restrict__ dinv, int ne, int relax_in,
-				  PrepLoads<R, NV> &P)
+				  const KinPar &kp, PrepLoads<R, NV> &P)
 {
 	const int t = lane_id();
 	if (relax_in) {
@@ -425,9 +432,85 @@ __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict
 		for (int u = 1; u < NV; u++)
 			P.fr[u] = (R)0;
 	}
+	if (LB && (d.flags & LB_FLAG)) {
+		/* the prediction (200 B) and the block's numbers instead of the
+		 * stored inverse (800 B) */
+		P.pw = t < 50 ? kp.lb_p[50L * it + t] : 0u;
+		P.d0 = t < 8 ? kp.lb_x[8L * it + t] : 0.0;
+		P.d1 = 0.0;
+		return;
+	}
 	const double *D = dinv + 100L * it;
 	P.d0 = D[t];
 	P.d1 = t < 36 ? D[64 + t] : 0.0;
+	P.pw = 0u;
+}
+
+/*
+ * D^-1 of a learned item into Dinv (LDS, rows padded to 11): D from its
+ * seven numbers, X0 = I + the network's prediction, then nsteps
+ * Newton-Schulz steps R = I - D X, X <- X + X R, every entry a sum over
+ * l = 0..9 in order (the set-up check does the same arithmetic). Lane t
+ * owns entries t and (t < 36) 64 + t. X and D live in scr (200 doubles:
+ * the G buffer of the item being prepared, written only afterwards), the
+ * numbers in xs (8 doubles).
+ */
+template <typename R, int NV>
+__device__ inline void learned_inverse(const PrepLoads<R, NV> &P, double (*Dinv)[11],
+				       double *scr, double *xs, int nsteps)
+{
+	const int t = threadIdx.x;
+	double *X = scr, *Dm = scr + 100;
+	const int ne_ = t < 36 ? 2 : 1; /* entries t and 64 + t */
+	if (t < 8)
+		xs[t] = P.d0;
+	if (t < 50) {
+		const int a = 2 * t, b = 2 * t + 1;
+		X[a] = lb_pred((unsigned short)(P.pw & 0xffffu)) + (a / 10 == a % 10 ? 1.0 : 0.0);
+		X[b] = lb_pred((unsigned short)(P.pw >> 16)) + (b / 10 == b % 10 ? 1.0 : 0.0);
+	}
+	__syncthreads();
+#pragma unroll 1
+	for (int m = 0; m < ne_; m++) {
+		const int e = t + 64 * m;
+		Dm[e] = lb_entry(xs, e / 10, e % 10);
+	}
+	__syncthreads();
+#pragma unroll 1
+	for (int step = 0; step < nsteps; step++) {
+		/* R = I - D X, kept in Dinv */
+#pragma unroll 1
+		for (int m = 0; m < ne_; m++) {
+			const int e = t + 64 * m, i = e / 10, j = e % 10;
+			double r = i == j ? 1.0 : 0.0;
+#pragma unroll
+			for (int l = 0; l < 10; l++)
+				r -= Dm[10 * i + l] * X[10 * l + j];
+			Dinv[i][j] = r;
+		}
+		__syncthreads();
+		/* X <- X + X R */
+		double xn[2];
+#pragma unroll 1
+		for (int m = 0; m < ne_; m++) {
+			const int e = t + 64 * m, i = e / 10, j = e % 10;
+			double x = X[e];
+#pragma unroll
+			for (int l = 0; l < 10; l++)
+				x += X[10 * i + l] * Dinv[l][j];
+			xn[m] = x;
+		}
+		__syncthreads();
+#pragma unroll 1
+		for (int m = 0; m < ne_; m++) {
+			const int e = t + 64 * m;
+			if (step + 1 < nsteps)
+				X[e] = xn[m];
+			else
+				Dinv[e / 10][e % 10] = xn[m];
+		}
+		__syncthreads();
+	}
 }
 
 /* u = the ghost value, or on a lagged face (1 - theta) f_nb + theta x_nb
@@ -436,7 +519,7 @@ __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict
  * state holds the previous transport's output) and rounded to R as a stored
  * state would be. Fills B (G_s and the slots) and fv (the lane's value of
  * the item, t < 60) and returns the lane's base value. */
-template <typename R, int NV>
+template <typename R, int NV, bool LB = false>
 __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> &P,
 				      const R *__restrict__ fo, const R *__restrict__ xl,
 				      int ne, int neq, int nv, int relax_in, const KinPar &kp,
@@ -446,9 +529,13 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 	fv = relax_in ? (R)relax_value<NV>(kp, P.fr, d.v, kp.ieps ? kp.ieps[d.e] : 1.0) : P.fr[0];
 	const int gf = given_faces(d), sf = slot_faces(d);
 	const int iw = t / 10, i = t % 10;
-	W.D[t / 10][t % 10] = P.d0;
-	if (t < 36)
-		W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
+	if (LB && (d.flags & LB_FLAG)) {
+		learned_inverse(P, W.D, &B.G[0][0][0], W.S, ((unsigned)d.flags >> 30) + 2);
+	} else {
+		W.D[t / 10][t % 10] = P.d0;
+		if (t < 36)
+			W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
+	}
 	if (t == 0) {
 		int s = 0;
 		B.v = d.v;
@@ -688,7 +775,7 @@ __device__ inline void load_tables(SweepTables &T)
 /* One upwind level per launch, one item per 64-lane block; the inflow
  * traces are the hand-off words of earlier levels (a word without the
  * sweep's epoch is reported in *status). */
-template <typename R, int NV>
+template <typename R, int NV, bool LB = false>
 __global__ __launch_bounds__(64) void k_sweep_level(
 	const ItemDesc *__restrict__ desc, int begin, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
@@ -704,9 +791,10 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	const int it = begin + blockIdx.x;
 	const ItemDesc d = desc[it];
 	PrepLoads<R, NV> P;
-	prep_issue(it, d, fo, dinv, ne, relax_in, P);
+	prep_issue<R, NV, LB>(it, d, fo, dinv, ne, relax_in, kp, P);
 	R fv;
-	const double base = prep_compute(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B, W, fv);
+	const double base =
+		prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B, W, fv);
 	const int t = threadIdx.x;
 	const unsigned pending =
 		ll_poll<R>(B, ll, ne, epoch, t < 36 ? (1u << B.nslot) - 1u : 0u, X);
@@ -744,8 +832,8 @@ __device__ inline ItemDesc desc_from_lanes(int x)
  * records a 100 us busy interval, and the largest number of overlapping
  * intervals is a set of blocks that were resident together.
  */
-template <typename R, bool FULL_POLL, int NV>
-__global__ __launch_bounds__(64, 5) void k_sweep_persistent(
+template <typename R, bool FULL_POLL, int NV, bool LB = false>
+__global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
 	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
@@ -780,8 +868,9 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 	{
 		const ItemDesc d = desc[it];
 		PrepLoads<R, NV> P;
-		prep_issue(it, d, fo, dinv, ne, relax_in, P);
-		base = prep_compute(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W, fv);
+		prep_issue<R, NV, LB>(it, d, fo, dinv, ne, relax_in, kp, P);
+		base = prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W,
+					       fv);
 	}
 	ItemDesc dn; /* descriptor of item it + G */
 	if (it + G < nitems)
@@ -791,7 +880,7 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 		const bool has_next = next < nitems;
 		PrepLoads<R, NV> P;
 		if (has_next)
-			prep_issue(next, dn, fo, dinv, ne, relax_in, P);
+			prep_issue<R, NV, LB>(next, dn, fo, dinv, ne, relax_in, kp, P);
 		/* descriptor of item next + G: one dword per lane (t < 20) */
 		int dnn = 0;
 		const int tl = lane_id();
@@ -804,8 +893,8 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 		if (__ballot(pending != 0)) {
 			/* prepare the next item, then watch one word per slot */
 			if (has_next)
-				nbase = prep_compute(dn, P, fo, xl, ne, neq, nv, relax_in, kp, T,
-						     B[k ^ 1], W, nfv);
+				nbase = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
+								kp, T, B[k ^ 1], W, nfv);
 			int spins = 0;
 			if (!FULL_POLL) {
 				bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
@@ -828,8 +917,8 @@ __global__ __launch_bounds__(64, 5) void k_sweep_persistent(
 			__syncthreads(); /* X complete */
 			sweep_finish<R>(B[k], X, fv, base, fn, ll, ne, epoch);
 			if (has_next)
-				nbase = prep_compute(dn, P, fo, xl, ne, neq, nv, relax_in, kp, T,
-						     B[k ^ 1], W, nfv);
+				nbase = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
+								kp, T, B[k ^ 1], W, nfv);
 		}
 		if (has_next) {
 			fv = nfv;
@@ -1272,14 +1361,23 @@ using SweepKernel = void (*)(const ItemDesc *, int, const R *, R *, const R *, c
 
 /* the persistent sweep of this context: precision, way of polling and
  * number of velocities */
-template <typename R> static SweepKernel<R> sweep_kernel(const gdh_ctx *c)
+template <typename R, bool LB> static SweepKernel<R> sweep_kernel_lb(const gdh_ctx *c)
 {
 	if (c->kp.nv == 4)
-		return c->sweep_poll ? k_sweep_persistent<R, true, 4> : k_sweep_persistent<R, false, 4>;
+		return c->sweep_poll ? k_sweep_persistent<R, true, 4, LB>
+				     : k_sweep_persistent<R, false, 4, LB>;
 	if (c->kp.nv == 6)
-		return c->sweep_poll ? k_sweep_persistent<R, true, 6> : k_sweep_persistent<R, false, 6>;
+		return c->sweep_poll ? k_sweep_persistent<R, true, 6, LB>
+				     : k_sweep_persistent<R, false, 6, LB>;
 	fprintf(stderr, "gdon3d-hip: no transport sweep for %d velocities\n", c->kp.nv);
 	abort();
+}
+
+/* the kernel with the learned refinement only where learned inverses are
+ * enabled: the refinement's registers stay out of the exact sweep */
+template <typename R> static SweepKernel<R> sweep_kernel(const gdh_ctx *c)
+{
+	return c->kp.lb_p ? sweep_kernel_lb<R, true>(c) : sweep_kernel_lb<R, false>(c);
 }
 
 template <typename R> static int sweep_residency(gdh_ctx *c)
@@ -1398,14 +1496,11 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 			if (!n)
 				continue;
 			c->prof_launches++;
-			if (nv == 4)
-				k_sweep_level<R, 4><<<n, 64, 0, c->stream>>>(
-					desc, b, fo, fn, xl, dinv, nv, ne, neq, ll, epoch, edev, status,
-					relax_in, kp);
-			else
-				k_sweep_level<R, 6><<<n, 64, 0, c->stream>>>(
-					desc, b, fo, fn, xl, dinv, nv, ne, neq, ll, epoch, edev, status,
-					relax_in, kp);
+			const bool lb = kp.lb_p != nullptr;
+			auto lk = nv == 4 ? (lb ? k_sweep_level<R, 4, true> : k_sweep_level<R, 4, false>)
+					  : (lb ? k_sweep_level<R, 6, true> : k_sweep_level<R, 6, false>);
+			lk<<<n, 64, 0, c->stream>>>(desc, b, fo, fn, xl, dinv, nv, ne, neq, ll, epoch,
+						    edev, status, relax_in, kp);
 		}
 		HIPCHECK(hipGetLastError());
 		return;
@@ -1956,6 +2051,37 @@ static void relax_choose(gdh_ctx *c)
 	 * step can be captured */
 	if (c->ll_partial && c->sweep_mode != 2)
 		ll_restart(c);
+}
+
+__global__ void k_flag_learned(ItemDesc *__restrict__ desc, const int *__restrict__ ok,
+			       int nitems)
+{
+	const int it = blockIdx.x * blockDim.x + threadIdx.x;
+	if (it >= nitems)
+		return;
+	int fl = desc[it].flags & ~(LB_FLAG | (3 << 30));
+	if (ok && ok[it] >= 2) /* refinement steps, 2 .. 5 */
+		fl |= LB_FLAG | ((ok[it] - 2) << 30);
+	desc[it].flags = fl;
+}
+
+/* descriptor flags of the items on the learned inverse, ok[it] their
+ * refinement steps (0 or null: the stored inverse);
+ * the learned inverses are refined by the item and level sweeps, so a
+ * context on the cluster sweep moves to the item sweep (the form of the
+ * relaxation chosen again by timing) */
+void sweep_set_learned(gdh_ctx *c, const int *ok)
+{
+	k_flag_learned<<<nblocks(c->nitems, 256), 256, 0, c->stream>>>(c->desc, ok, c->nitems);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	/* the kernel of the item sweep changes with kp.lb_p: its residency is
+	 * measured again */
+	c->sweep_grid = grid_cap(c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c));
+	if (ok && c->sweep_mode == 2)
+		c->sweep_mode = 1;
+	if (c->sweep_mode != 0)
+		relax_choose(c);
 }
 
 extern "C" int gdh_relax_fused(gdh_ctx *c)

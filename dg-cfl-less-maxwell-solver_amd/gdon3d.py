@@ -190,6 +190,7 @@ def load_library(path=LIB_PATH):
         "gdh_rk3_info": (None, [C.c_void_p, C.POINTER(C.c_int)]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
+        "gdh_learned_enable": (None, [C.c_void_p, C.c_void_p, C.c_double, c_double_p]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
         "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
         "t10_get_inv_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
@@ -581,6 +582,19 @@ def learned_solve_stats(stepper, net, idx, tol=1e-8, maxit=50):
     lib.gdh_learned_solve_stats(ctx, net.h if net is not None else None, len(idx), iptr(idx),
                                 tol, maxit, dptr(out), None)
     return out
+
+
+def learned_enable(stepper, net, tol=1e-8):
+    """Put the sweep's block inverses on the network's refined predictions
+    (gdh_learned_enable; net None: back on the stored exact inverses).
+    Returns (items on the network, items on the stored inverse, largest
+    relative deviation of a refined inverse from the stored one, median
+    max|I - D X0| of the predictions)."""
+    lib = load_library()
+    out = np.zeros(4)
+    lib.gdh_learned_enable(lib.dg_stepper_device_context(stepper.h),
+                           net.h if net is not None else None, tol, dptr(out))
+    return int(out[0]), int(out[1]), float(out[2]), float(out[3])
 
 
 def normalization(x, y):

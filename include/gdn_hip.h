@@ -338,6 +338,22 @@ void gdh_learned_data(gdh_ctx *ctx, int n, const int *idx, float *x, float *y);
 void gdh_learned_solve_stats(gdh_ctx *ctx, gdh_mlp *mlp, int n, const int *idx,
 			     double tol, int maxit, double *out, int *iters);
 
+/* Learned block inverses in the transport sweep: the network mlp predicts
+ * D^-1 - I of every sweep item from its seven numbers; the predictions are
+ * stored in bfloat16 (200 B per item, where the exact binary64 inverse takes
+ * 800 B), and the item and level sweeps rebuild D from the numbers and
+ * refine the prediction by Newton-Schulz steps (R = I - D X,
+ * X <- X + X R) to the binary64 inverse. An item whose residual before the
+ * last step exceeds tol keeps the stored exact inverse (the fallback, and
+ * the judge: out[1] counts them). A context on the cluster sweep moves to
+ * the item sweep. out[0] items on the network's inverse, out[1] on the
+ * stored one, out[2] largest max|X - D^-1| / max|D^-1| of the refined
+ * inverses against the stored ones, out[3] median max|I - D X0| of the
+ * predictions. mlp NULL: every item back on the stored inverse. No
+ * counterpart in the reference (KLU factors every block exactly,
+ * src/maths/sparse.c:411-466). */
+void gdh_learned_enable(gdh_ctx *ctx, gdh_mlp *mlp, double tol, double out[4]);
+
 /* Copies `bytes` (read) into a second buffer (written) `reps` times with a
  * plain streaming kernel (k_copy_calib) after one warm-up launch, at 8, 16,
  * 32 and 64 blocks per CU; returns the fastest mean milliseconds per launch. No counterpart in the reference: the
