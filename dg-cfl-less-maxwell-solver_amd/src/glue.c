@@ -409,6 +409,75 @@ double *gdn_eval_imposed_ghosts(const gdn_simulation *simu, double t)
 	return w;
 }
 
+/*
+ * One device context kept for the standalone entry points
+ * (dg_init_sol_macro, dg_update_boundary, dg_apply_relaxation), which a
+ * reference-style driver may call every stage: the context of the last
+ * simulation they served, reused while the simulation, its mesh, state
+ * length, scheme and model (including the exact-solution parameters) are
+ * the same; each call uploads simu->wn and downloads it again.
+ * simulation_free drops it.
+ */
+static struct {
+	const gdn_simulation *simu;
+	const gdn_mesh *mesh;
+	/* the mesh's sizes and arrays: a struct reused for another mesh
+	 * without simulation_free does not match */
+	int nbelems, nbnodes, nboundaryfaces;
+	const void *node, *elem2node, *face2elem;
+	int wlen, kinetic;
+	gdh_model_desc md;
+	gdh_ctx *ctx;
+} g_ctx_cache;
+
+static int ctx_cache_matches(const gdn_simulation *simu, const gdh_model_desc *md)
+{
+	const gdn_mesh *m = simu->tt;
+	return g_ctx_cache.ctx && g_ctx_cache.simu == simu && g_ctx_cache.mesh == m &&
+	       g_ctx_cache.nbelems == m->nbelems && g_ctx_cache.nbnodes == m->nbnodes &&
+	       g_ctx_cache.nboundaryfaces == m->nboundaryfaces &&
+	       g_ctx_cache.node == (const void *)m->node &&
+	       g_ctx_cache.elem2node == (const void *)m->elem2node &&
+	       g_ctx_cache.face2elem == (const void *)m->face2elem &&
+	       g_ctx_cache.wlen == simu->wlen &&
+	       g_ctx_cache.kinetic == (simu->use_kinetic_scheme ? 1 : 0) &&
+	       !memcmp(&g_ctx_cache.md, md, sizeof(*md));
+}
+
+gdh_ctx *gdn_ctx_cached(gdn_simulation *simu, gdh_model_desc *md)
+{
+	if (gdh_device_count() < 1)
+		gdn_fatal("no HIP device is visible; the GPU solver cannot run");
+	gdn_model_describe(simu->mdl, simu->use_kinetic_scheme, md);
+	if (ctx_cache_matches(simu, md)) {
+		gdh_upload_state(g_ctx_cache.ctx, simu->wn);
+		return g_ctx_cache.ctx;
+	}
+	gdn_ctx_cache_drop(NULL);
+	gdh_model_desc m2;
+	g_ctx_cache.ctx = gdn_ctx_open(simu, &m2);
+	g_ctx_cache.simu = simu;
+	g_ctx_cache.mesh = simu->tt;
+	g_ctx_cache.nbelems = simu->tt->nbelems;
+	g_ctx_cache.nbnodes = simu->tt->nbnodes;
+	g_ctx_cache.nboundaryfaces = simu->tt->nboundaryfaces;
+	g_ctx_cache.node = simu->tt->node;
+	g_ctx_cache.elem2node = simu->tt->elem2node;
+	g_ctx_cache.face2elem = simu->tt->face2elem;
+	g_ctx_cache.wlen = simu->wlen;
+	g_ctx_cache.kinetic = simu->use_kinetic_scheme ? 1 : 0;
+	g_ctx_cache.md = *md;
+	return g_ctx_cache.ctx;
+}
+
+void gdn_ctx_cache_drop(const gdn_simulation *simu)
+{
+	if (g_ctx_cache.ctx && (!simu || g_ctx_cache.simu == simu)) {
+		gdh_destroy(g_ctx_cache.ctx);
+		memset(&g_ctx_cache, 0, sizeof(g_ctx_cache));
+	}
+}
+
 gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md)
 {
 	return gdn_ctx_open_ex(simu, md, GDH_FP64);

@@ -43,6 +43,11 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 /* A device context for the simulation with its state uploaded (and the
  * imposed-data arrays of GDH_IMPOSED_HOST helpers). */
 gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md);
+/* the context of the standalone entry points, kept between calls for the
+ * same simulation (state uploaded at every call); do not destroy it */
+gdh_ctx *gdn_ctx_cached(gdn_simulation *simu, gdh_model_desc *md);
+/* drop the kept context (of simu, or any when simu is NULL) */
+void gdn_ctx_cache_drop(const gdn_simulation *simu);
 /* the same with the device state precision (GDH_FP64 or GDH_FP32) */
 gdh_ctx *gdn_ctx_open_ex(gdn_simulation *simu, gdh_model_desc *md, int precision);
 

@@ -81,6 +81,7 @@ void simulation_init(gdn_mesh *msh, gdn_model *mdl, gdn_simulation *f)
 /* src/simulation/simulation.c:193-209: frees wn and the mesh arrays */
 void simulation_free(gdn_simulation *f)
 {
+	gdn_ctx_cache_drop(f); /* the standalone entry points' device context */
 	f->m = 0;
 	f->wlen = 0;
 	f->neq = 0;
