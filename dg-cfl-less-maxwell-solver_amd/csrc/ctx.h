@@ -114,12 +114,18 @@ struct gdh_ctx {
 	void *wz; /* lagged values z, scattered into a state-shaped buffer */
 	int lag_last, lag_max; /* sweeps per step: last step, largest */
 	/* capacitance form over the cycles: distinct lagged nodes (ipg and
-	 * velocity), per-velocity offsets, LU of (I - T S) column-major */
+	 * velocity), per-velocity offsets, the inverse of the row-permuted
+	 * (I - T S) column-major with its row permutation, partial products */
 	int cap_qtot;
 	std::vector<int> cap_off_h;
 	int *cap_ipg, *cap_vel, *cap_off, *cap_piv;
-	double *cap_lu, *cap_t;
+	double *cap_lu, *cap_t, *cap_part, *cap_rk;
 	long *cap_luoff;
+	/* matrix-free form (more lagged nodes than the dense LU takes): GMRES
+	 * on the same systems, started from the last step's lagged values */
+	int cap_gmres;
+	std::vector<double> cap_zh;
+	int cap_warned; /* GMRES stopped above its tolerance (reported once) */
 	int cap_qmax() const
 	{
 		int m = 0;

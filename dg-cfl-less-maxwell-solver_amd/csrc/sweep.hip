@@ -13,7 +13,11 @@ using namespace gdh;
 /* ===================================================================== */
 
 #define THETA 0.5 /* Crank-Nicolson, src/dg/relaxation.c:133 */
-#define CAP_MAX_NODES 4096 /* lagged nodes of one velocity (dense LU) */
+/* lagged nodes of one velocity taken by the dense inverse of the
+ * capacitance form (GDN_CAP_MAX_NODES overrides, up to CAP_MAX_DENSE: q^2
+ * binary64 values per velocity); above, the matrix-free form */
+#define CAP_MAX_NODES 4096
+#define CAP_MAX_DENSE 16384
 
 static void cap_setup(gdh_ctx *c);
 static void relax_choose(gdh_ctx *c);
@@ -1701,8 +1705,11 @@ void relax_launch(gdh_ctx *c, void *f, long n)
  * producers' face nodes, z = T x, so (I - T S) z = T x0: a small dense system
  * per velocity (q = distinct lagged nodes), the same for the 6 fields. S is
  * built once at set-up from sweeps with unit lagged inputs (6 columns per
- * sweep, one per field slot) and (I - T S) is LU-factored on the host in
- * binary64. Each step: x0 = sweep(f, 0); t = T x0; z = (I - T S)^-1 t;
+ * sweep, one per field slot, written on the device), and (I - T S) is
+ * inverted there in binary64 by Gauss-Jordan elimination with partial
+ * pivoting, in place: the result is X = (P (I - T S))^-1 for the row
+ * permutation P of the pivots, so z = X (P t). Each step: x0 = sweep(f, 0);
+ * t = T x0; z = X (P t), a dense product for the 6 fields at once;
  * x = sweep(f, z) — two sweeps, exact up to rounding.
  */
 template <typename R>
@@ -1717,41 +1724,65 @@ __global__ void k_cap_gather(const R *__restrict__ x, const int *__restrict__ ip
 	t[k] = (double)x[kin_at(ne, neq, nv, ipg[r], iw, vel[r])];
 }
 
-/* z = (I - T S)_v^-1 t for one (v, iw) per block: LU column-major with row
- * pivots; the result is scattered into the lagged-value state buffer */
-template <typename R>
-__global__ __launch_bounds__(64) void k_cap_solve(
-	const double *__restrict__ lu, const long *__restrict__ luoff,
-	const int *__restrict__ piv, const int *__restrict__ off,
-	const int *__restrict__ ipg, const double *__restrict__ t, int qtot, int nv,
-	long ne, long neq, R *__restrict__ wz)
+#define CAP_JC 256 /* columns of X per workgroup of the product */
+
+/* partial products of z_v = X_v (P t)_v over the columns [c CAP_JC,
+ * (c + 1) CAP_JC) of X_v, the 6 fields at once; one row per lane, X read
+ * column by column (coalesced); grid (rows / 256, column chunks, nv) */
+__global__ __launch_bounds__(256) void k_cap_product(
+	const double *__restrict__ X, const long *__restrict__ luoff, const int *__restrict__ off,
+	const int *__restrict__ piv, const double *__restrict__ t, int qtot, int qmax,
+	double *__restrict__ part)
 {
-	extern __shared__ double y[];
-	const int v = blockIdx.x / 6, iw = blockIdx.x % 6;
+	__shared__ double y[6][CAP_JC];
+	const int v = blockIdx.z, c = blockIdx.y;
 	const int o = off[v], q = off[v + 1] - o;
-	const double *A = lu + luoff[v];
-	const int *p = piv + o;
-	const double *tv = t + (long)iw * qtot + o;
-	for (int i = threadIdx.x; i < q; i += 64)
-		y[i] = tv[p[i]];
+	const int j0 = c * CAP_JC;
+	if (j0 >= q)
+		return;
+	const int nj = min(q - j0, CAP_JC);
+	for (int j = threadIdx.x; j < nj; j += 256) {
+		const int pj = piv[o + j0 + j];
+#pragma unroll
+		for (int iw = 0; iw < 6; iw++)
+			y[iw][j] = t[(long)iw * qtot + o + pj];
+	}
 	__syncthreads();
-	for (int j = 0; j < q; j++) { /* unit lower triangle */
-		const double yj = y[j];
-		for (int i = j + 1 + threadIdx.x; i < q; i += 64)
-			y[i] -= A[(long)j * q + i] * yj;
-		__syncthreads();
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= q)
+		return;
+	const double *A = X + luoff[v] + (long)j0 * q + i;
+	double acc[6] = { 0, 0, 0, 0, 0, 0 };
+	for (int j = 0; j < nj; j++) {
+		const double a = A[(long)j * q];
+#pragma unroll
+		for (int iw = 0; iw < 6; iw++)
+			acc[iw] += a * y[iw][j];
 	}
-	for (int j = q - 1; j >= 0; j--) { /* upper triangle */
-		if (threadIdx.x == 0)
-			y[j] /= A[(long)j * q + j];
-		__syncthreads();
-		const double yj = y[j];
-		for (int i = threadIdx.x; i < j; i += 64)
-			y[i] -= A[(long)j * q + i] * yj;
-		__syncthreads();
-	}
-	for (int i = threadIdx.x; i < q; i += 64)
-		wz[kin_at(ne, neq, nv, ipg[o + i], iw, v)] = (R)y[i];
+	const int nch = (qmax + CAP_JC - 1) / CAP_JC;
+#pragma unroll
+	for (int iw = 0; iw < 6; iw++)
+		part[(((long)v * nch + c) * 6 + iw) * qmax + i] = acc[iw];
+}
+
+/* z: the sum of the partial products (fixed order), scattered into the
+ * lagged-value state buffer; grid (6 qmax / 256, nv) */
+template <typename R>
+__global__ void k_cap_reduce(const double *__restrict__ part, const int *__restrict__ off,
+			     const int *__restrict__ ipg, int qmax, int nv, long ne, long neq,
+			     R *__restrict__ wz)
+{
+	const int v = blockIdx.y;
+	const int o = off[v], q = off[v + 1] - o;
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 6 * q)
+		return;
+	const int iw = k / q, i = k % q;
+	const int nc = (q + CAP_JC - 1) / CAP_JC, nch = (qmax + CAP_JC - 1) / CAP_JC;
+	double s = 0;
+	for (int c = 0; c < nc; c++)
+		s += part[(((long)v * nch + c) * 6 + iw) * qmax + i];
+	wz[kin_at(ne, neq, nv, ipg[o + i], iw, v)] = (R)s;
 }
 
 /* unit lagged inputs of set-up batch c: field slot iw carries column
@@ -1768,6 +1799,121 @@ __global__ void k_cap_unit(R *__restrict__ wz, const int *__restrict__ ipg,
 		wz[kin_at(ne, neq, nv, ipg[off[v] + r], iw, v)] = (R)1;
 }
 
+/* columns 6 batch + iw of (I - T S)_v from the gathered T S e (column-major);
+ * grid (6 qmax / 256, nv) */
+__global__ void k_cap_column(const double *__restrict__ t, const int *__restrict__ off,
+			     const long *__restrict__ luoff, int batch, int qtot,
+			     double *__restrict__ X)
+{
+	const int v = blockIdx.y;
+	const int o = off[v], q = off[v + 1] - o;
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 6 * q)
+		return;
+	const int iw = k / q, i = k % q, col = 6 * batch + iw;
+	if (col < q)
+		X[luoff[v] + (long)col * q + i] =
+			(i == col ? 1.0 : 0.0) - t[(long)iw * qtot + o + i];
+}
+
+/* Step k of the in-place Gauss-Jordan inversion of every X_v (one
+ * workgroup per velocity): the row p >= k of the largest |X_v(p, k)| is
+ * swapped with row k (piv records it), then row k and column k are saved
+ * (rk: row k, then column k, 2 qmax per velocity) for the update; a zero
+ * or non-finite pivot sets *sing. */
+__global__ __launch_bounds__(1024) void k_gj_pivot(double *__restrict__ X,
+						   const long *__restrict__ luoff,
+						   const int *__restrict__ off, int *__restrict__ piv,
+						   double *__restrict__ rk, int qmax, int k,
+						   int *__restrict__ sing)
+{
+	__shared__ double bv[1024];
+	__shared__ int bi[1024];
+	const int v = blockIdx.x, tid = threadIdx.x;
+	const int o = off[v], q = off[v + 1] - o;
+	if (k >= q)
+		return;
+	double *A = X + luoff[v];
+	double best = -1.0;
+	int bidx = k;
+	for (int i = k + tid; i < q; i += 1024) {
+		const double a = fabs(A[(long)k * q + i]);
+		if (a > best) {
+			best = a;
+			bidx = i;
+		}
+	}
+	bv[tid] = best;
+	bi[tid] = bidx;
+	__syncthreads();
+	for (int s = 512; s > 0; s >>= 1) {
+		if (tid < s && (bv[tid + s] > bv[tid] ||
+				(bv[tid + s] == bv[tid] && bi[tid + s] < bi[tid]))) {
+			bv[tid] = bv[tid + s];
+			bi[tid] = bi[tid + s];
+		}
+		__syncthreads();
+	}
+	const int p = bi[0];
+	if (p != k) {
+		for (int j = tid; j < q; j += 1024) {
+			const double a = A[(long)j * q + k];
+			A[(long)j * q + k] = A[(long)j * q + p];
+			A[(long)j * q + p] = a;
+		}
+		if (tid == 0) {
+			const int s = piv[o + k];
+			piv[o + k] = piv[o + p];
+			piv[o + p] = s;
+		}
+	}
+	__syncthreads();
+	double *r = rk + (long)v * 2 * qmax;
+	for (int j = tid; j < q; j += 1024) {
+		r[j] = A[(long)j * q + k];
+		r[qmax + j] = A[(long)k * q + j];
+	}
+	if (tid == 0) {
+		const double pv = A[(long)k * q + k];
+		if (pv == 0.0 || !isfinite(pv))
+			*sing = 1;
+	}
+}
+
+#define GJ_JC 16 /* columns per lane in the Gauss-Jordan update */
+
+/* the rest of step k: every entry of X_v from its old value and the saved
+ * row k (r) and column k (cc): X(k, k) = 1 / p, X(k, j) = r_j / p,
+ * X(i, k) = -cc_i / p, X(i, j) -= cc_i r_j / p; grid (rows / 256,
+ * columns / GJ_JC, nv) */
+__global__ __launch_bounds__(256) void k_gj_update(double *__restrict__ X,
+						   const long *__restrict__ luoff,
+						   const int *__restrict__ off,
+						   const double *__restrict__ rk, int qmax, int k)
+{
+	const int v = blockIdx.z;
+	const int o = off[v], q = off[v + 1] - o;
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	const int j0 = blockIdx.y * GJ_JC;
+	if (k >= q || i >= q || j0 >= q)
+		return;
+	const int j1 = min(q, j0 + GJ_JC);
+	const double *r = rk + (long)v * 2 * qmax, *cc = r + qmax;
+	const double inv = 1.0 / cc[k], ci = cc[i];
+	double *A = X + luoff[v];
+	for (int j = j0; j < j1; j++) {
+		const double rj = r[j] * inv;
+		double a;
+		if (i == k)
+			a = j == k ? inv : rj;
+		else if (j == k)
+			a = -ci * inv;
+		else
+			a = A[(long)j * q + i] - ci * rj;
+		A[(long)j * q + i] = a;
+	}
+}
+
 static void cap_gather(gdh_ctx *c, const void *x)
 {
 	const int q = c->cap_qtot;
@@ -1780,40 +1926,8 @@ static void cap_gather(gdh_ctx *c, const void *x)
 	HIPCHECK(hipGetLastError());
 }
 
-/* partial-pivoting LU of the n x n column-major matrix a, in place */
-static void lu_factor(double *a, int n, int *piv)
-{
-	for (int i = 0; i < n; i++)
-		piv[i] = i;
-	for (int k = 0; k < n; k++) {
-		int p = k;
-		for (int i = k + 1; i < n; i++)
-			if (fabs(a[(long)k * n + i]) > fabs(a[(long)k * n + p]))
-				p = i;
-		if (a[(long)k * n + p] == 0.0) {
-			fprintf(stderr, "gdon3d-hip: singular capacitance matrix of the "
-					"transport over upwind cycles\n");
-			abort();
-		}
-		if (p != k) {
-			for (int j = 0; j < n; j++)
-				std::swap(a[(long)j * n + k], a[(long)j * n + p]);
-			std::swap(piv[k], piv[p]);
-		}
-		const double d = a[(long)k * n + k];
-		for (int i = k + 1; i < n; i++)
-			a[(long)k * n + i] /= d;
-		for (int j = k + 1; j < n; j++) {
-			const double akj = a[(long)j * n + k];
-			if (akj == 0.0)
-				continue;
-			for (int i = k + 1; i < n; i++)
-				a[(long)j * n + i] -= a[(long)k * n + i] * akj;
-		}
-	}
-}
-
-/* Set-up of the capacitance form: positions, S by unit sweeps, LU. */
+/* Set-up of the capacitance form: positions, (I - T S) by unit sweeps, its
+ * inverse (or, above the node limit, the matrix-free form's start). */
 static void cap_setup(gdh_ctx *c)
 {
 	const int nv = c->kp.nv;
@@ -1841,11 +1955,10 @@ static void cap_setup(gdh_ctx *c)
 	int qmax = 0;
 	for (int v = 0; v < nv; v++)
 		qmax = std::max(qmax, off[v + 1] - off[v]);
-	if (qmax > CAP_MAX_NODES) {
-		fprintf(stderr, "gdon3d-hip: %d lagged nodes in one velocity's upwind "
-				"cycles (limit %d)\n", qmax, CAP_MAX_NODES);
-		abort();
-	}
+	int qcap = CAP_MAX_NODES;
+	if (const char *env = getenv("GDN_CAP_MAX_NODES"))
+		qcap = std::min(std::max(atoi(env), 0), CAP_MAX_DENSE);
+	c->cap_gmres = qmax > qcap;
 	c->cap_qtot = qtot;
 	c->cap_off_h = off;
 	c->cap_ipg = (int *)dmalloc(sizeof(int) * qtot);
@@ -1854,21 +1967,34 @@ static void cap_setup(gdh_ctx *c)
 	HIPCHECK(hipMemcpy(c->cap_ipg, ipg.data(), sizeof(int) * qtot, hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(c->cap_vel, vel.data(), sizeof(int) * qtot, hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(c->cap_off, off.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice));
-	c->cap_t = (double *)dmalloc(sizeof(double) * 6 * qtot);
+	c->cap_t = (double *)dmalloc(sizeof(double) * 6 * (qtot > 0 ? qtot : 1));
 
 	/* S columns: sweeps of zero data with unit lagged inputs */
 	const size_t blen = (size_t)c->m * c->neq;
 	c->wz = dmalloc(c->rsz * blen);
 	void *zero = c->wx, *out = c->w[1 - c->cur];
 	HIPCHECK(hipMemsetAsync(zero, 0, c->rsz * blen, c->stream));
-	std::vector<std::vector<double>> M(nv);
+	if (c->cap_gmres) {
+		HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
+		c->cap_zh.assign(6 * (size_t)qtot, 0.0);
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		return;
+	}
+	std::vector<long> luoff(nv + 1, 0);
+	std::vector<int> piv(qtot > 0 ? qtot : 1);
 	for (int v = 0; v < nv; v++) {
 		const int q = off[v + 1] - off[v];
-		M[v].assign((size_t)q * q, 0.0);
-		for (int i = 0; i < q; i++)
-			M[v][(size_t)i * q + i] = 1.0;
+		luoff[v + 1] = luoff[v] + (long)q * q;
+		for (int r = 0; r < q; r++)
+			piv[off[v] + r] = r;
 	}
-	std::vector<double> th(6 * (size_t)qtot);
+	c->cap_lu = (double *)dmalloc(sizeof(double) * (luoff[nv] > 0 ? luoff[nv] : 1));
+	c->cap_luoff = (long *)dmalloc(sizeof(long) * (nv + 1));
+	HIPCHECK(hipMemcpy(c->cap_luoff, luoff.data(), sizeof(long) * (nv + 1),
+			   hipMemcpyHostToDevice));
+	c->cap_piv = (int *)dmalloc(sizeof(int) * piv.size());
+	HIPCHECK(hipMemcpy(c->cap_piv, piv.data(), sizeof(int) * piv.size(),
+			   hipMemcpyHostToDevice));
 	const int nbatch = (qmax + 5) / 6;
 	for (int bt = 0; bt < nbatch; bt++) {
 		HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
@@ -1881,55 +2007,254 @@ static void cap_setup(gdh_ctx *c)
 		HIPCHECK(hipGetLastError());
 		sweep_step(c, zero, out, c->wz, 0);
 		cap_gather(c, out);
-		HIPCHECK(hipMemcpyAsync(th.data(), c->cap_t, sizeof(double) * th.size(),
-					hipMemcpyDeviceToHost, c->stream));
-		HIPCHECK(hipStreamSynchronize(c->stream));
-		for (int v = 0; v < nv; v++) {
-			const int q = off[v + 1] - off[v];
-			for (int iw = 0; iw < 6; iw++) {
-				const int col = 6 * bt + iw;
-				if (col >= q)
+		k_cap_column<<<dim3(nblocks(6 * qmax, 256), nv), 256, 0, c->stream>>>(
+			c->cap_t, c->cap_off, c->cap_luoff, bt, qtot, c->cap_lu);
+		HIPCHECK(hipGetLastError());
+	}
+	HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
+	/* X = (P (I - T S))^-1 in place */
+	c->cap_rk = (double *)dmalloc(sizeof(double) * 2 * (size_t)qmax * nv);
+	int *sing = (int *)dmalloc(sizeof(int));
+	HIPCHECK(hipMemsetAsync(sing, 0, sizeof(int), c->stream));
+	const dim3 ug(nblocks(qmax, 256), (qmax + GJ_JC - 1) / GJ_JC, nv);
+	for (int k = 0; k < qmax; k++) {
+		k_gj_pivot<<<nv, 1024, 0, c->stream>>>(c->cap_lu, c->cap_luoff, c->cap_off,
+							c->cap_piv, c->cap_rk, qmax, k, sing);
+		k_gj_update<<<ug, 256, 0, c->stream>>>(c->cap_lu, c->cap_luoff, c->cap_off,
+						       c->cap_rk, qmax, k);
+	}
+	HIPCHECK(hipGetLastError());
+	int hsing = 0;
+	HIPCHECK(hipMemcpyAsync(&hsing, sing, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	HIPCHECK(hipFree(sing));
+	if (hsing) {
+		fprintf(stderr, "gdon3d-hip: singular capacitance matrix of the "
+				"transport over upwind cycles\n");
+		abort();
+	}
+	const int nch = (qmax + CAP_JC - 1) / CAP_JC;
+	c->cap_part = (double *)dmalloc(sizeof(double) * (size_t)nv * nch * 6 * qmax);
+}
+
+/*
+ * Matrix-free form of the same solve, for velocities with more lagged nodes
+ * than the dense inverse takes: restarted GMRES (Saad and Schultz, 1986; modified
+ * Gram-Schmidt, Givens rotations) on (I - T S) z = T x0. The operator is
+ * block diagonal, one system per (velocity, field) as above, so the systems
+ * run in lockstep, each with its own Krylov basis and Hessenberg matrix,
+ * and one product S z for all of them is one sweep of zero data with
+ * lagged inputs z. The Krylov vectors (6 q values each) live on the host;
+ * each iteration moves them through the device once. Started from the last
+ * step's lagged values; stops when every system's true residual is at most
+ * tol |(T x0)_k| + 1e-16 max_k |(T x0)_k| (tol 1e-14 in binary64, 2e-6 in
+ * binary32), when a restart cycle no longer lowers it (rounding level), or
+ * after GMRES_MAX_SWEEPS products.
+ */
+#define GMRES_RESTART 30
+#define GMRES_MAX_SWEEPS 1200
+
+template <typename R>
+__global__ void k_cap_scatter(const double *__restrict__ z, const int *__restrict__ ipg,
+			      const int *__restrict__ vel, int q, int nv, long ne, long neq,
+			      R *__restrict__ wz)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= 6 * q)
+		return;
+	const int iw = k / q, r = k % q;
+	wz[kin_at(ne, neq, nv, ipg[r], iw, vel[r])] = (R)z[k];
+}
+
+/* z (host, 6 q values, field-major) into the lagged-value buffer */
+static void cap_put(gdh_ctx *c, const double *z)
+{
+	const int q = c->cap_qtot;
+	HIPCHECK(hipMemcpyAsync(c->cap_t, z, sizeof(double) * 6 * q, hipMemcpyHostToDevice,
+				c->stream));
+	if (c->fp32)
+		k_cap_scatter<float><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
+			c->cap_t, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->ne, c->neq, (float *)c->wz);
+	else
+		k_cap_scatter<double><<<nblocks(6 * q, 256), 256, 0, c->stream>>>(
+			c->cap_t, c->cap_ipg, c->cap_vel, q, c->kp.nv, c->ne, c->neq, (double *)c->wz);
+	HIPCHECK(hipGetLastError());
+}
+
+/* y = (I - T S) z for every system; out: a state buffer to sweep into */
+static void cap_apply(gdh_ctx *c, const double *z, double *y, void *out)
+{
+	const size_t n = 6 * (size_t)c->cap_qtot;
+	cap_put(c, z);
+	sweep_step(c, c->wx, out, c->wz, 0);
+	cap_gather(c, out);
+	HIPCHECK(hipMemcpyAsync(y, c->cap_t, sizeof(double) * n, hipMemcpyDeviceToHost,
+				c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	for (size_t k = 0; k < n; k++)
+		y[k] = z[k] - y[k];
+}
+
+/* The lagged values from T x0 (in cap_t), left in wz; returns the number
+ * of sweeps made. out: a state buffer to sweep into. */
+static int cap_gmres(gdh_ctx *c, void *out)
+{
+	const int nv = c->kp.nv, q = c->cap_qtot, nb = 6 * nv, M = GMRES_RESTART;
+	const size_t n = 6 * (size_t)q;
+	const std::vector<int> &off = c->cap_off_h;
+	std::vector<double> b(n), r(n), w(n), V((size_t)(M + 1) * n);
+	HIPCHECK(hipMemcpyAsync(b.data(), c->cap_t, sizeof(double) * n, hipMemcpyDeviceToHost,
+				c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	std::vector<double> &x = c->cap_zh;
+	/* system k = iw nv + v: values [iw q + off[v], iw q + off[v + 1]) */
+	auto lo = [&](int k) { return (size_t)(k / nv) * q + off[k % nv]; };
+	auto hi = [&](int k) { return (size_t)(k / nv) * q + off[k % nv + 1]; };
+	auto dot = [&](const double *a, const double *d, int k) {
+		double s = 0;
+		for (size_t i = lo(k); i < hi(k); i++)
+			s += a[i] * d[i];
+		return s;
+	};
+	const double tol = c->fp32 ? 2e-6 : 1e-14;
+	std::vector<double> tolk(nb), beta(nb), H((size_t)nb * (M + 1) * M), cs((size_t)nb * M),
+		sn((size_t)nb * M), g((size_t)nb * (M + 1));
+	std::vector<int> ncol(nb);
+	std::vector<char> act(nb);
+	double bmax = 0;
+	for (int k = 0; k < nb; k++) {
+		tolk[k] = std::sqrt(dot(b.data(), b.data(), k));
+		bmax = std::max(bmax, tolk[k]);
+	}
+	if (bmax == 0) { /* T x0 = 0: z = 0 */
+		std::fill(x.begin(), x.end(), 0.0);
+		cap_put(c, x.data());
+		return 0;
+	}
+	for (int k = 0; k < nb; k++)
+		tolk[k] = tol * tolk[k] + 1e-16 * bmax;
+	int sweeps = 0;
+	double prev = HUGE_VAL;
+	for (;;) {
+		cap_apply(c, x.data(), r.data(), out);
+		sweeps++;
+		int nact = 0;
+		double worst = 0; /* largest residual relative to its tolerance */
+		for (int k = 0; k < nb; k++) {
+			for (size_t i = lo(k); i < hi(k); i++)
+				r[i] = b[i] - r[i];
+			beta[k] = std::sqrt(dot(r.data(), r.data(), k));
+			act[k] = beta[k] > tolk[k];
+			nact += act[k];
+			if (tolk[k] > 0)
+				worst = std::max(worst, beta[k] / tolk[k]);
+		}
+		if (!nact)
+			break;
+		if (sweeps >= GMRES_MAX_SWEEPS || worst > 0.9 * prev) {
+			if (!c->cap_warned)
+				fprintf(stderr, "gdon3d-hip: GMRES over the upwind cycles stopped "
+						"at %.2g times its tolerance after %d sweeps\n",
+					worst, sweeps);
+			c->cap_warned = 1;
+			break;
+		}
+		prev = worst;
+		std::fill(V.begin(), V.begin() + n, 0.0);
+		for (int k = 0; k < nb; k++) {
+			ncol[k] = 0;
+			std::fill(g.begin() + (size_t)k * (M + 1), g.begin() + (size_t)(k + 1) * (M + 1),
+				  0.0);
+			if (!act[k])
+				continue;
+			g[(size_t)k * (M + 1)] = beta[k];
+			for (size_t i = lo(k); i < hi(k); i++)
+				V[i] = r[i] / beta[k];
+		}
+		for (int j = 0; j < M && nact; j++) {
+			double *vj = V.data() + (size_t)j * n, *vn = vj + n;
+			cap_apply(c, vj, w.data(), out);
+			sweeps++;
+			std::fill(vn, vn + n, 0.0);
+			for (int k = 0; k < nb; k++) {
+				if (!act[k])
 					continue;
-				for (int i = 0; i < q; i++) /* column-major */
-					M[v][(size_t)col * q + i] -= th[(size_t)iw * qtot + off[v] + i];
+				double *h = H.data() + (size_t)k * (M + 1) * M; /* h[i M + j] */
+				for (int i = 0; i <= j; i++) {
+					const double *vi = V.data() + (size_t)i * n;
+					const double hij = dot(w.data(), vi, k);
+					h[i * M + j] = hij;
+					for (size_t l = lo(k); l < hi(k); l++)
+						w[l] -= hij * vi[l];
+				}
+				const double hn = std::sqrt(dot(w.data(), w.data(), k));
+				h[(j + 1) * M + j] = hn;
+				if (hn > 0)
+					for (size_t l = lo(k); l < hi(k); l++)
+						vn[l] = w[l] / hn;
+				double *ck = cs.data() + (size_t)k * M, *sk = sn.data() + (size_t)k * M;
+				double *gk = g.data() + (size_t)k * (M + 1);
+				for (int i = 0; i < j; i++) {
+					const double a = h[i * M + j], d = h[(i + 1) * M + j];
+					h[i * M + j] = ck[i] * a + sk[i] * d;
+					h[(i + 1) * M + j] = -sk[i] * a + ck[i] * d;
+				}
+				const double a = h[j * M + j], d = h[(j + 1) * M + j];
+				const double rho = std::hypot(a, d);
+				ck[j] = rho > 0 ? a / rho : 1.0;
+				sk[j] = rho > 0 ? d / rho : 0.0;
+				h[j * M + j] = rho;
+				h[(j + 1) * M + j] = 0;
+				gk[j + 1] = -sk[j] * gk[j];
+				gk[j] = ck[j] * gk[j];
+				ncol[k] = j + 1;
+				if (std::fabs(gk[j + 1]) <= tolk[k] || hn == 0) {
+					act[k] = 0;
+					nact--;
+				}
+			}
+		}
+		/* x_k += V_k y_k with H_k y_k = g_k (upper triangular) */
+		for (int k = 0; k < nb; k++) {
+			const int nc = ncol[k];
+			if (!nc)
+				continue;
+			const double *h = H.data() + (size_t)k * (M + 1) * M;
+			double y[GMRES_RESTART];
+			for (int i = nc - 1; i >= 0; i--) {
+				double s = g[(size_t)k * (M + 1) + i];
+				for (int l = i + 1; l < nc; l++)
+					s -= h[i * M + l] * y[l];
+				y[i] = s / h[i * M + i];
+			}
+			for (int i = 0; i < nc; i++) {
+				const double *vi = V.data() + (size_t)i * n;
+				for (size_t l = lo(k); l < hi(k); l++)
+					x[l] += y[i] * vi[l];
 			}
 		}
 	}
-	HIPCHECK(hipMemsetAsync(c->wz, 0, c->rsz * blen, c->stream));
-	std::vector<long> luoff(nv + 1, 0);
-	std::vector<int> piv(qtot > 0 ? qtot : 1);
-	for (int v = 0; v < nv; v++) {
-		const int q = off[v + 1] - off[v];
-		lu_factor(M[v].data(), q, piv.data() + off[v]);
-		luoff[v + 1] = luoff[v] + (long)q * q;
-	}
-	c->cap_lu = (double *)dmalloc(sizeof(double) * (luoff[nv] > 0 ? luoff[nv] : 1));
-	for (int v = 0; v < nv; v++)
-		if (luoff[v + 1] > luoff[v])
-			HIPCHECK(hipMemcpy(c->cap_lu + luoff[v], M[v].data(),
-					   sizeof(double) * (luoff[v + 1] - luoff[v]),
-					   hipMemcpyHostToDevice));
-	c->cap_luoff = (long *)dmalloc(sizeof(long) * (nv + 1));
-	HIPCHECK(hipMemcpy(c->cap_luoff, luoff.data(), sizeof(long) * (nv + 1),
-			   hipMemcpyHostToDevice));
-	c->cap_piv = (int *)dmalloc(sizeof(int) * piv.size());
-	HIPCHECK(hipMemcpy(c->cap_piv, piv.data(), sizeof(int) * piv.size(),
-			   hipMemcpyHostToDevice));
-	HIPCHECK(hipStreamSynchronize(c->stream));
+	cap_put(c, x.data());
+	return sweeps;
 }
 
 static void cap_solve(gdh_ctx *c)
 {
-	const int nv = c->kp.nv;
-	const size_t lds = sizeof(double) * std::max(1, c->cap_qmax());
+	const int nv = c->kp.nv, qmax = c->cap_qmax();
+	const int nch = (qmax + CAP_JC - 1) / CAP_JC;
+	k_cap_product<<<dim3(nblocks(qmax, 256), nch, nv), 256, 0, c->stream>>>(
+		c->cap_lu, c->cap_luoff, c->cap_off, c->cap_piv, c->cap_t, c->cap_qtot, qmax,
+		c->cap_part);
+	HIPCHECK(hipGetLastError());
+	const dim3 rg(nblocks(6 * qmax, 256), nv);
 	if (c->fp32)
-		k_cap_solve<float><<<6 * nv, 64, lds, c->stream>>>(
-			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
-			c->cap_qtot, nv, c->ne, c->neq, (float *)c->wz);
+		k_cap_reduce<float><<<rg, 256, 0, c->stream>>>(c->cap_part, c->cap_off, c->cap_ipg,
+							      qmax, nv, c->ne, c->neq,
+							      (float *)c->wz);
 	else
-		k_cap_solve<double><<<6 * nv, 64, lds, c->stream>>>(
-			c->cap_lu, c->cap_luoff, c->cap_piv, c->cap_off, c->cap_ipg, c->cap_t,
-			c->cap_qtot, nv, c->ne, c->neq, (double *)c->wz);
+		k_cap_reduce<double><<<rg, 256, 0, c->stream>>>(c->cap_part, c->cap_off,
+							       c->cap_ipg, qmax, nv, c->ne,
+							       c->neq, (double *)c->wz);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -1960,13 +2285,18 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 	if (!c->nlagged) {
 		sweep_step(c, fo, fn, nullptr, rin);
 	} else {
-		/* exact over the upwind cycles: two sweeps and a small solve */
+		/* exact over the upwind cycles: two sweeps around a solve for
+		 * the lagged values (dense inverse, or GMRES with sweeps as products) */
 		sweep_step(c, fo, fn, nullptr, rin);
 		cap_gather(c, fn);
-		cap_solve(c);
+		int extra = 0;
+		if (c->cap_gmres)
+			extra = cap_gmres(c, fn);
+		else
+			cap_solve(c);
 		sweep_step(c, fo, fn, c->wz, rin);
-		c->lag_last = 2;
-		c->lag_max = 2;
+		c->lag_last = 2 + extra;
+		c->lag_max = std::max(c->lag_max, c->lag_last);
 	}
 	prof_end(c, 0);
 	c->relax_pending = 1;

@@ -1354,6 +1354,7 @@ typedef struct {
 	csr M1[8], M2[8];
 	blocklu LU[8];
 	int assembled;
+	int nofactor; /* kinetic: assemble M1, M2 without factoring M1 */
 } osim;
 
 void *orc_sim_create(void *mesh, int scheme, int imposed_kind,
@@ -1406,6 +1407,17 @@ void orc_sim_set_eps(void *h, const double *eps)
 void orc_sim_set_fr_projector(void *h, int on)
 {
 	((osim *)h)->fr_projector = on;
+}
+
+/* 1: the kinetic assembly skips the factorization of M1 (residual checks
+ * on meshes whose strong components are too large for dense blocks); the
+ * solves then refuse to run */
+void orc_sim_set_nofactor(void *h, int on)
+{
+	osim *s = h;
+	if (s->assembled)
+		die("orc_sim_set_nofactor after assembly");
+	s->nofactor = on;
 }
 
 void orc_sim_set_threads(void *h, int n)
@@ -1752,7 +1764,8 @@ static void assemble(osim *s)
 			spgemm_shift(&Mi, &Fv, 1.0, theta * s->dt, &s->M1[iv]);
 			spgemm_shift(&Mi, &Fv, 1.0, -(1 - theta) * s->dt, &s->M2[iv]);
 			csr_free(&Fv);
-			blocklu_factor(&s->M1[iv], &s->LU[iv]);
+			if (!s->nofactor)
+				blocklu_factor(&s->M1[iv], &s->LU[iv]);
 		}
 		csr_free(&Mi);
 	}
@@ -1832,6 +1845,8 @@ static void solve_rk3(osim *s, int nsteps)
 static void solve_relax(osim *s, int nsteps)
 {
 	double t0 = now();
+	if (s->nofactor)
+		die("the kinetic solve needs the factorization of M1");
 	assemble(s);
 	s->t_asm = now() - t0;
 	init_sol(s);
@@ -1903,6 +1918,8 @@ void orc_rk3_operator(void *h, const double *w, double *dw)
 void orc_transport(void *h, int iv, const double *f, double *sol)
 {
 	osim *s = h;
+	if (s->nofactor)
+		die("orc_transport needs the factorization of M1");
 	tables_init();
 	set_threads(s);
 	assemble(s);
@@ -1932,6 +1949,8 @@ double orc_transport_residual(void *h, int iv, const double *f, const double *x)
 void orc_block_stats(void *h, int iv, int *out)
 {
 	osim *s = h;
+	if (s->nofactor)
+		die("orc_block_stats needs the factorization of M1");
 	assemble(s);
 	const blocklu *F = &s->LU[iv];
 	int mx = 0;

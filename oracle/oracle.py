@@ -58,6 +58,7 @@ def lib():
             "orc_error_l2": (C.c_double, [C.c_void_p]),
             "orc_energy": (C.c_double, [C.c_void_p]),
             "orc_sim_set_fr_projector": (None, [C.c_void_p, C.c_int]),
+            "orc_sim_set_nofactor": (None, [C.c_void_p, C.c_int]),
             "orc_sim_set_eps": (None, [C.c_void_p, dp]),
             "orc_proj": (None, [dp, dp, C.c_int, dp]),
             "orc_fr_projector": (None, [C.c_int, dp, dp, dp, dp]),
@@ -119,7 +120,10 @@ class OSim:
     """Assembled-matrix solve of the reference algorithm on an OMesh."""
 
     def __init__(self, mesh, scheme, imposed_kind, imposed_par=None, kin=0,
-                 bflux=UPWIND, lam=None, threads=None, fr_projector=False, eps=None):
+                 bflux=UPWIND, lam=None, threads=None, fr_projector=False, eps=None,
+                 factor=True):
+        """factor=False: the kinetic assembly leaves M1 unfactored (only
+        transport_residual and the products are available)."""
         par = np.zeros(16)
         if imposed_par is not None:
             par[: len(imposed_par)] = imposed_par
@@ -129,6 +133,7 @@ class OSim:
             lib().orc_sim_set_lambda(self.h, lam)
         lib().orc_sim_set_threads(self.h, threads or os.cpu_count() or 1)
         lib().orc_sim_set_fr_projector(self.h, int(fr_projector))
+        lib().orc_sim_set_nofactor(self.h, int(not factor))
         if eps is not None:
             self._eps = np.ascontiguousarray(eps, dtype=np.float64)
             assert self._eps.shape == (mesh.nelem,)
