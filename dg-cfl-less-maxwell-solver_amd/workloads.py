@@ -12,8 +12,11 @@ def batch_instances(count=256, seed=5):
     [4, 24], a plane wave of random direction d on the unit sphere with
     |k| = 2 pi m, m in {1, 2, 3}, E0 a random unit vector orthogonal to k,
     and eps uniform in [1, 4] per 4^3 block of cubes. Returns a list of
-    (n, par[16], eps[6 n^3]) with the plane-wave parameter block of speed 1
-    (the instances' exact data; with eps != 1 it is not the solution)."""
+    (n, par[16], eps[6 n^3]) with the plane-wave parameter block: for the
+    cubes of one block (n <= 4, uniform eps) the exact solution, the wave of
+    speed 1/sqrt(eps) with H = sqrt(eps) d x E; for the others the wave of
+    speed 1 (initial and boundary data only: with eps jumping between
+    blocks there is no closed-form solution)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -30,6 +33,9 @@ def batch_instances(count=256, seed=5):
         table = rng.uniform(1.0, 4.0, size=(nb, nb, nb))
         cube = np.arange(6 * n ** 3) // 6  # 6 tets per cube, cube i + n (j + n k)
         i, j, kk = cube % n, (cube // n) % n, cube // (n * n)
+        if nb == 1:
+            par[6:9] *= np.sqrt(table[0, 0, 0])
+            par[9] /= np.sqrt(table[0, 0, 0])
         out.append((n, par, table[i // 4, j // 4, kk // 4]))
     return out
 
