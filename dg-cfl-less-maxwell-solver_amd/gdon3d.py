@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libgdon3d_hip.so")
+# GDN_LIB: another build of the same library (the host-sanitizer build,
+# `make sanitize`: lib-san/libgdon3d_hip.so)
+LIB_PATH = os.environ.get("GDN_LIB") or os.path.join(HERE, "lib", "libgdon3d_hip.so")
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+# GDN_ORACLE_LIB: another build of oracle.c (`make -C oracle sanitize`)
+ORACLE_LIB = os.environ.get("GDN_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libgdon3d_refsubset.so")
 
 dp = C.POINTER(C.c_double)
