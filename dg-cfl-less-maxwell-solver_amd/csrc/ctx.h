@@ -92,6 +92,9 @@ struct gdh_ctx {
 	unsigned long long *ll; /* per item: 60 hand-off words, one per value in the
 				 * state precision, epoch parity in the lowest bit */
 	int ll_partial; /* the last sweep left some items' words unwritten */
+	unsigned *queue; /* item counters of the persistent sweep (two slots) */
+	int qslot; /* the slot of the next launch */
+	int sweep_queue; /* 1: items taken from the counter, 0: static order */
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */

@@ -271,6 +271,9 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->cap_lu = c->cap_t = c->cap_part = c->cap_rk = nullptr;
 	c->cap_luoff = nullptr;
 	c->cap_gmres = 0;
+	c->queue = nullptr;
+	c->qslot = 0;
+	c->sweep_queue = 0;
 	c->cap_warned = 0;
 	c->nlagged = 0;
 	c->lag_last = c->lag_max = 0;
@@ -306,7 +309,7 @@ extern "C" void gdh_destroy(gdh_ctx *c)
 			 c->hostvals, c->partials, c->items, c->dinv,
 			 c->cls,   c->desc,  c->ll, c->status, c->gh, c->geom_soa, c->elem_pos,
 			 c->wx, c->wz, c->cap_ipg, c->cap_vel, c->cap_off, c->cap_piv,
-			 c->cap_lu, c->cap_t, c->cap_part, c->cap_rk, c->cap_luoff, c->epoch_dev, c->chead, c->cdesc,
+			 c->cap_lu, c->cap_t, c->cap_part, c->cap_rk, c->cap_luoff, c->queue, c->epoch_dev, c->chead, c->cdesc,
 			 c->bt_mem[0], c->bt_mem[1], c->bt_mem[2], c->bt_mem[3],
 			 c->bt_dmem[0], c->bt_dmem[1], c->bt_dmem[2], c->bt_dmem[3],
 			 c->stage, c->bface, c->ieps, c->lb_x, c->lb_p };

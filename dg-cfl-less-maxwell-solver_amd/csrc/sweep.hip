@@ -836,13 +836,41 @@ __device__ inline ItemDesc desc_from_lanes(int x)
  * records a 100 us busy interval, and the largest number of overlapping
  * intervals is a set of blocks that were resident together.
  */
-template <typename R, bool FULL_POLL, int NV, bool LB = false>
+/* the next index of the sweep's item queue (lane 0's request; the caller
+ * makes it uniform where it needs it) */
+__device__ inline unsigned queue_take(unsigned *q)
+{
+	unsigned r = 0;
+	if (threadIdx.x == 0)
+		r = __hip_atomic_fetch_add(q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return r;
+}
+
+/*
+ * With QU the blocks take their items from counters instead of in the
+ * order b, b + G, ...: the items are dealt into S = min(QUEUE_SHARDS, G)
+ * shards (item i to shard i mod S), block b serves shard b mod S, and the
+ * blocks of a shard take its items in level order from the shard's counter
+ * (one counter per cache line; slot `slot` of two, zeroed by the launch
+ * before, which zeroes the other slot for the next one). A block that is
+ * not resident takes nothing, and a block takes its items in increasing
+ * order holding at most its current one and the two after it, so the
+ * earliest unfinished item of a shard is a running block's current item
+ * as long as one block of every shard runs: the first S blocks of the
+ * grid suffice, not the whole grid. Each block knows two items ahead: it
+ * prepares the next one, loads the descriptor of the one after, and asks
+ * the counter for a third.
+ */
+#define QUEUE_SHARDS 64
+#define QUEUE_STRIDE 32 /* unsigned words between counters: 128 B */
+#define QUEUE_BYTES (2 * QUEUE_SHARDS * QUEUE_STRIDE * sizeof(unsigned))
+template <typename R, bool FULL_POLL, int NV, bool LB = false, bool QU = false>
 __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
 	int nv, int ne, int neq, u64 *ll, unsigned epoch_arg,
 	const unsigned *__restrict__ epoch_dev, int *status, u64 *disc, int relax_in,
-	KinPar kp)
+	KinPar kp, unsigned *queue, int slot)
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
@@ -864,7 +892,25 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 	/* the sweep's epoch: an argument, or (captured graphs) device memory */
 	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
 	const int G = gridDim.x;
-	int it = blockIdx.x, k = 0;
+	const int S = min(QUEUE_SHARDS, G), shard = blockIdx.x % S;
+	unsigned *q = QU ? queue + (slot * QUEUE_SHARDS + shard) * QUEUE_STRIDE : nullptr;
+	if (QU && blockIdx.x == 0 && t < QUEUE_SHARDS)
+		__hip_atomic_store(queue + ((1 - slot) * QUEUE_SHARDS + t) * QUEUE_STRIDE, 0u,
+				   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	/* the current item, the next one and the one after it */
+	int it, next, nn;
+	if (QU) {
+		it = shard + S * (int)__builtin_amdgcn_readfirstlane(queue_take(q));
+		next = it < nitems ? shard + S * (int)__builtin_amdgcn_readfirstlane(queue_take(q))
+				   : nitems;
+		nn = next < nitems ? shard + S * (int)__builtin_amdgcn_readfirstlane(queue_take(q))
+				   : nitems;
+	} else {
+		it = blockIdx.x;
+		next = it + G;
+		nn = next + G;
+	}
+	int k = 0;
 	if (it >= nitems)
 		return;
 	R fv; /* the lane's old and base values of the current item */
@@ -876,20 +922,21 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 		base = prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W,
 					       fv);
 	}
-	ItemDesc dn; /* descriptor of item it + G */
-	if (it + G < nitems)
-		dn = desc[it + G];
-	for (; it < nitems; it += G, k ^= 1) {
-		const int next = it + G;
+	ItemDesc dn; /* descriptor of the next item */
+	if (next < nitems)
+		dn = desc[next];
+	for (; it < nitems; k ^= 1) {
 		const bool has_next = next < nitems;
 		PrepLoads<R, NV> P;
 		if (has_next)
 			prep_issue<R, NV, LB>(next, dn, fo, dinv, ne, relax_in, kp, P);
-		/* descriptor of item next + G: one dword per lane (t < 20) */
+		/* descriptor of the item after next: one dword per lane (t < 20);
+		 * with the queue, the index of the one after that */
 		int dnn = 0;
 		const int tl = lane_id();
-		if (next + G < nitems && tl < (int)(sizeof(ItemDesc) / 4))
-			dnn = reinterpret_cast<const int *>(desc + next + G)[tl];
+		if (nn < nitems && tl < (int)(sizeof(ItemDesc) / 4))
+			dnn = reinterpret_cast<const int *>(desc + nn)[tl];
+		const unsigned nnn_req = QU && nn < nitems ? queue_take(q) : 0u;
 		double nbase = 0.0;
 		R nfv = (R)0;
 		unsigned pending = t < 36 ? (1u << uni(B[k].nslot)) - 1u : 0u;
@@ -928,8 +975,15 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 			fv = nfv;
 			base = nbase;
 		}
-		if (next + G < nitems)
+		const int nnn = !QU ? nn + G
+				   : nn < nitems
+					   ? shard + S * (int)__builtin_amdgcn_readfirstlane(nnn_req)
+					   : nitems;
+		if (nn < nitems)
 			dn = desc_from_lanes(dnn);
+		it = next;
+		next = nn;
+		nn = nnn;
 	}
 }
 
@@ -1361,20 +1415,27 @@ static void cluster_setup(gdh_ctx *c, const uint8_t *cls, const int *items_ve,
 template <typename R>
 using SweepKernel = void (*)(const ItemDesc *, int, const R *, R *, const R *, const double *,
 			     int, int, int, u64 *, unsigned, const unsigned *, int *, u64 *, int,
-			     KinPar);
+			     KinPar, unsigned *, int);
 
 /* the persistent sweep of this context: precision, way of polling and
- * number of velocities */
-template <typename R, bool LB> static SweepKernel<R> sweep_kernel_lb(const gdh_ctx *c)
+ * number of velocities ... */
+template <typename R, bool LB, bool QU>
+static SweepKernel<R> sweep_kernel_q(const gdh_ctx *c)
 {
 	if (c->kp.nv == 4)
-		return c->sweep_poll ? k_sweep_persistent<R, true, 4, LB>
-				     : k_sweep_persistent<R, false, 4, LB>;
+		return c->sweep_poll ? k_sweep_persistent<R, true, 4, LB, QU>
+				     : k_sweep_persistent<R, false, 4, LB, QU>;
 	if (c->kp.nv == 6)
-		return c->sweep_poll ? k_sweep_persistent<R, true, 6, LB>
-				     : k_sweep_persistent<R, false, 6, LB>;
+		return c->sweep_poll ? k_sweep_persistent<R, true, 6, LB, QU>
+				     : k_sweep_persistent<R, false, 6, LB, QU>;
 	fprintf(stderr, "gdon3d-hip: no transport sweep for %d velocities\n", c->kp.nv);
 	abort();
+}
+
+/* ... and the way its blocks take their items (static order or queue) */
+template <typename R, bool LB> static SweepKernel<R> sweep_kernel_lb(const gdh_ctx *c)
+{
+	return c->sweep_queue ? sweep_kernel_q<R, LB, true>(c) : sweep_kernel_q<R, LB, false>(c);
 }
 
 /* the kernel with the learned refinement only where learned inverses are
@@ -1394,7 +1455,7 @@ template <typename R> static int sweep_residency(gdh_ctx *c)
 	const int g0 = std::max(1, per_cu * ncu);
 	u64 *disc = (u64 *)dmalloc(sizeof(u64) * 2 * (size_t)g0);
 	kern<<<g0, 64, 0, c->stream>>>(nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
-				       nullptr, 0, nullptr, nullptr, disc, 0, c->kp);
+				       nullptr, 0, nullptr, nullptr, disc, 0, c->kp, nullptr, 0);
 	HIPCHECK(hipGetLastError());
 	std::vector<u64> h(2 * (size_t)g0);
 	HIPCHECK(hipMemcpyAsync(h.data(), disc, sizeof(u64) * h.size(),
@@ -1418,6 +1479,14 @@ void sweep_setup(gdh_ctx *c)
 	c->ll_partial = 0;
 	c->status = (int *)dmalloc(sizeof(int));
 	HIPCHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+	c->queue = (unsigned *)dmalloc(QUEUE_BYTES);
+	HIPCHECK(hipMemsetAsync(c->queue, 0, QUEUE_BYTES, c->stream));
+	c->qslot = 0;
+	/* GDN_SWEEP_QUEUE=static|dynamic: the persistent sweep's items in the
+	 * order b, b + G, ... (needs every block resident: cooperative launch)
+	 * or taken from a counter (any grid) */
+	const char *qenv = getenv("GDN_SWEEP_QUEUE");
+	c->sweep_queue = qenv ? !strcmp(qenv, "dynamic") : 0;
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	c->epoch = 0;
 	/* GDN_SWEEP=level|item (or persistent)|cluster fixes the form of the sweep; by default
@@ -1457,6 +1526,8 @@ static void ll_restart(gdh_ctx *c)
 		abort();
 	}
 	HIPCHECK(hipMemsetAsync(c->ll, 0, c->rsz * 60 * (size_t)c->nitems, c->stream));
+	HIPCHECK(hipMemsetAsync(c->queue, 0, QUEUE_BYTES, c->stream));
+	c->qslot = 0;
 	if (c->epoch_dev)
 		HIPCHECK(hipMemsetAsync(c->epoch_dev, 0, sizeof(unsigned), c->stream));
 	c->epoch = 0;
@@ -1535,16 +1606,23 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 	}
 	const int grid = std::min(c->sweep_grid, nitems);
 	const SweepKernel<R> kern = sweep_kernel<R>(c);
-	if (edev) {
-		/* an ordinary launch with the measured resident grid, since a
-		 * captured stream takes no cooperative launch */
+	/* items from the queue (slots alternate between launches) or in the
+	 * static order */
+	unsigned *queue = c->sweep_queue ? c->queue : nullptr;
+	int slot = c->qslot;
+	if (queue)
+		c->qslot = 1 - c->qslot;
+	if (edev || queue) {
+		/* an ordinary launch with the measured resident grid: a captured
+		 * stream takes no cooperative launch, and the queue needs no
+		 * co-residency */
 		kern<<<grid, 64, 0, c->stream>>>(desc, nitems, fo_c, fn, xl, dinv, nv, ne, neq, ll,
-						 0u, edev, status, disc, relax_in, kp);
+						 epoch, edev, status, disc, relax_in, kp, queue, slot);
 		HIPCHECK(hipGetLastError());
 		return;
 	}
 	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
-			 &epoch, &edev, &status, &disc, &relax_in, &kp };
+			 &epoch, &edev, &status, &disc, &relax_in, &kp, &queue, &slot };
 	HIPCHECK(hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(64), args, 0,
 					    c->stream));
 }

@@ -360,3 +360,23 @@ def test_relax_projector_boundary_state(kin, form, precision, monkeypatch):
     assert ran == (sweep, "0")
     assert rel_l2(other, ref) > 1e-6
     assert r <= (TOL if precision == "fp64" else 2e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("kin,n,jit", [("d3q4", 6, 0.0), ("d3q6", 4, 0.15)])
+def test_item_queue_equals_static_order(kin, n, jit, fused, monkeypatch):
+    """GDN_SWEEP_QUEUE=dynamic: the item sweep's blocks take their items
+    from sharded counters instead of the static order b, b + G, ...; every
+    item computes the same values whichever block takes it, so the whole
+    solve is bit-identical to the static order (itself checked against the
+    oracle above), D3Q4 and D3Q6, relaxation fused or not."""
+    path = kuhn_msh(n, jit, 11)
+    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    out = {}
+    for q in ("static", "dynamic"):
+        monkeypatch.setenv("GDN_SWEEP_QUEUE", q)
+        out[q] = gpu_run(path, "cavity_mode", kin, 0.3, 5.0, ("item", fused), monkeypatch)
+    print(f"queue {kin} n={n} fused {fused}: ran {out['dynamic']['form']}", flush=True)
+    assert out["static"]["form"] == out["dynamic"]["form"] == ("item", fused)
+    assert np.array_equal(out["static"]["wn"], out["dynamic"]["wn"])
