@@ -127,7 +127,7 @@ struct gdh_ctx {
 	/* matrix-free form (more lagged nodes than the dense LU takes): GMRES
 	 * on the same systems, started from the last step's lagged values */
 	int cap_gmres;
-	std::vector<double> cap_zh;
+	std::vector<double> cap_zh, cap_krylov;
 	int cap_warned; /* GMRES stopped above its tolerance (reported once) */
 	int cap_qmax() const
 	{

@@ -2180,7 +2180,12 @@ static int cap_gmres(gdh_ctx *c, void *out)
 	const int nv = c->kp.nv, q = c->cap_qtot, nb = 6 * nv, M = GMRES_RESTART;
 	const size_t n = 6 * (size_t)q;
 	const std::vector<int> &off = c->cap_off_h;
-	std::vector<double> b(n), r(n), w(n), V((size_t)(M + 1) * n);
+	/* the Krylov basis is kept by the context (no zeroing per call: every
+	 * vector is written before it is read) */
+	std::vector<double> &V = c->cap_krylov;
+	if (V.size() < (size_t)(M + 1) * n)
+		V.resize((size_t)(M + 1) * n);
+	std::vector<double> b(n), r(n), w(n);
 	HIPCHECK(hipMemcpyAsync(b.data(), c->cap_t, sizeof(double) * n, hipMemcpyDeviceToHost,
 				c->stream));
 	HIPCHECK(hipStreamSynchronize(c->stream));
