@@ -16,10 +16,11 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Rebuild the library and the oracle whenever make finds them older
+    """Rebuild the library, the oracle and the C driver whenever make finds them older
     than their sources (make -q: nothing runs when they are up to date), so
     that a stale library is never what gets tested."""
-    for d, extra in ((PKG, ["-j8"]), (os.path.join(ROOT, "oracle"), [])):
+    for d, extra in ((PKG, ["-j8"]), (os.path.join(ROOT, "oracle"), []),
+                     (os.path.join(ROOT, "tests", "c"), [])):
         if subprocess.run(["make", "-q", "-C", d], stdout=subprocess.DEVNULL,
                           stderr=subprocess.DEVNULL).returncode != 0:
             subprocess.run(["make", "-C", d] + extra, check=True)
