@@ -7,7 +7,12 @@
  * O(N) instead of the reference's qsort passes, producing the same face
  * order, the same neighbour pairing and the same global T10 node numbering
  * (first appearance when scanning elements and local nodes in order, which is
- * what the reference's stable coordinate sort yields).
+ * what the reference's stable coordinate sort yields). The matching runs on
+ * the host's threads (OpenMP): the keys are dealt into one partition per
+ * thread by their hash, each thread scans every element in order and
+ * handles the keys of its partition, so every key sees its occurrences in
+ * the sequential order and the numbering is the sequential one (prefix
+ * sums where it counts); the arrays are bit-identical for any thread count.
  */
 #define _GNU_SOURCE
 #include <assert.h>
@@ -21,6 +26,27 @@
 #include <mesh/mesh.h>
 #include <mesh/t10.h>
 #include <mesh/tetrageometry.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+static int nparts(void)
+{
+	return omp_get_max_threads();
+}
+static int part_id(void)
+{
+	return omp_get_thread_num();
+}
+#else
+static int nparts(void)
+{
+	return 1;
+}
+static int part_id(void)
+{
+	return 0;
+}
+#endif
 
 static void die(const char *what)
 {
@@ -369,6 +395,12 @@ static size_t imap_hash(int a, int b, int c)
 	return (size_t)x;
 }
 
+/* partition (of np) of key (a,b,c): bits the slot index does not use */
+static int imap_part(int a, int b, int c, int np)
+{
+	return (int)((imap_hash(a, b, c) >> 40) % (uint64_t)np);
+}
+
 /* returns the slot of (a,b,c); *found tells whether it was present */
 static size_t imap_slot(imap *h, int a, int b, int c, int *found)
 {
@@ -418,33 +450,46 @@ static void build_elem2elem(gdn_mesh *m)
 	m->elem2elem = xcalloc(NFA * (size_t)ne, sizeof(int));
 	for (int i = 0; i < NFA * ne; i++)
 		m->elem2elem[i] = -1;
-	imap h;
-	imap_init(&h, (size_t)NFA * ne);
-	int nfaces = 0;
-	for (int e = 0; e < ne; e++) {
-		for (int f = 0; f < NFA; f++) {
-			int v[3];
-			for (int k = 0; k < NVE; k++)
-				v[k] = m->elem2node[NNO * e +
-						    t4_get_ref_face2node_entry(f, k)];
-			sort3(v);
-			int found;
-			const size_t s = imap_slot(&h, v[0], v[1], v[2], &found);
-			if (!found) {
-				memcpy(h.keys + 3 * s, v, sizeof(v));
-				h.vals[s] = NFA * e + f;
-				nfaces++;
-			} else {
-				const int other = h.vals[s];
-				if (other < 0)
-					die("a face is shared by more than two elements");
-				m->elem2elem[NFA * e + f] = other / NFA;
-				m->elem2elem[other] = e;
-				h.vals[s] = -1;
+	const int np = nparts();
+	int nfaces = 0, bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(np) reduction(+ : nfaces, bad)
+#endif
+	{
+		const int p = part_id();
+		imap h;
+		imap_init(&h, (size_t)NFA * ne / np + 16);
+		for (int e = 0; e < ne; e++) {
+			for (int f = 0; f < NFA; f++) {
+				int v[3];
+				for (int k = 0; k < NVE; k++)
+					v[k] = m->elem2node[NNO * e +
+							    t4_get_ref_face2node_entry(f, k)];
+				sort3(v);
+				if (imap_part(v[0], v[1], v[2], np) != p)
+					continue;
+				int found;
+				const size_t s = imap_slot(&h, v[0], v[1], v[2], &found);
+				if (!found) {
+					memcpy(h.keys + 3 * s, v, sizeof(v));
+					h.vals[s] = NFA * e + f;
+					nfaces++;
+				} else {
+					const int other = h.vals[s];
+					if (other < 0) {
+						bad++;
+						continue;
+					}
+					m->elem2elem[NFA * e + f] = other / NFA;
+					m->elem2elem[other] = e;
+					h.vals[s] = -1;
+				}
 			}
 		}
+		imap_free(&h);
 	}
-	imap_free(&h);
+	if (bad)
+		die("a face is shared by more than two elements");
 	m->nbfaces = nfaces;
 }
 
@@ -500,104 +545,156 @@ static void build_face_lists(gdn_mesh *m)
 }
 
 /* T10 nodes, numbered by first appearance (src/mesh/mesh.c:1018-1115);
- * coordinates are the affine images of the reference nodes. */
+ * coordinates are the affine images of the reference nodes. Position
+ * pos = NQN e + i: each partition records, per position, the first position
+ * of its key; the first positions, counted in order, give the numbers. */
 static void build_qnodes(gdn_mesh *m)
 {
 	static const int edge_v[6][2] = { { 0, 1 }, { 1, 2 }, { 0, 2 },
 					   { 0, 3 }, { 1, 3 }, { 2, 3 } };
 	const int ne = m->nbelems;
-	m->elem2qnode = xcalloc(NQN * (size_t)ne, sizeof(int));
-	gdn_real *xq = xcalloc(3 * (size_t)NQN * ne, sizeof(gdn_real));
-	imap h;
-	imap_init(&h, (size_t)(NNO + 6) * ne);
-	int nq = 0;
+	const size_t npos = (size_t)NQN * ne;
+	int *first = xcalloc(npos, sizeof(int)); /* the key's first position */
+	int *id = xcalloc(npos, sizeof(int)); /* 1 at first positions, then numbers */
+	const int np = nparts();
+#ifdef _OPENMP
+#pragma omp parallel num_threads(np)
+#endif
+	{
+		const int p = part_id();
+		imap h;
+		imap_init(&h, npos / np + 16);
+		for (int e = 0; e < ne; e++) {
+			const int *vid = m->elem2node + NNO * e;
+			for (int i = 0; i < NQN; i++) {
+				int key[3];
+				if (i < NNO) {
+					key[0] = vid[i];
+					key[1] = key[2] = -2;
+				} else {
+					int a = vid[edge_v[i - NNO][0]], b = vid[edge_v[i - NNO][1]];
+					key[0] = a < b ? a : b;
+					key[1] = a < b ? b : a;
+					key[2] = -3;
+				}
+				if (imap_part(key[0], key[1], key[2], np) != p)
+					continue;
+				const int pos = NQN * e + i;
+				int found;
+				const size_t s = imap_slot(&h, key[0], key[1], key[2], &found);
+				if (!found) {
+					memcpy(h.keys + 3 * s, key, sizeof(key));
+					h.vals[s] = pos;
+					id[pos] = 1;
+				}
+				first[pos] = h.vals[s];
+			}
+		}
+		imap_free(&h);
+	}
+	int nq = 0; /* numbers of the first positions, in order */
+	for (size_t pos = 0; pos < npos; pos++) {
+		const int is_first = id[pos];
+		id[pos] = nq;
+		nq += is_first;
+	}
+	m->elem2qnode = xcalloc(npos, sizeof(int));
+	gdn_real *xq = xcalloc(3 * (size_t)(nq > 0 ? nq : 1), sizeof(gdn_real));
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(np) schedule(static)
+#endif
 	for (int e = 0; e < ne; e++) {
 		const int *vid = m->elem2node + NNO * e;
-		gdn_real t4[NNO][3];
-		for (int v = 0; v < NNO; v++)
-			for (int d = 0; d < 3; d++)
-				t4[v][d] = m->node[3 * vid[v] + d];
 		for (int i = 0; i < NQN; i++) {
-			int key[3];
-			if (i < NNO) {
-				key[0] = vid[i];
-				key[1] = key[2] = -2;
-			} else {
-				int a = vid[edge_v[i - NNO][0]], b = vid[edge_v[i - NNO][1]];
-				key[0] = a < b ? a : b;
-				key[1] = a < b ? b : a;
-				key[2] = -3;
-			}
-			int found;
-			const size_t s = imap_slot(&h, key[0], key[1], key[2], &found);
-			if (!found) {
-				memcpy(h.keys + 3 * s, key, sizeof(key));
-				h.vals[s] = nq;
-				gdn_real xr[3];
-				t10_get_ref_node_coord(i, xr);
-				gdn_tetra_ref2phy(t4, xr, NULL, 0, xq + 3 * nq, NULL,
-						  NULL, NULL, NULL);
-				nq++;
-			}
-			m->elem2qnode[NQN * e + i] = h.vals[s];
+			const int pos = NQN * e + i;
+			m->elem2qnode[pos] = id[first[pos]];
+			if (first[pos] != pos)
+				continue;
+			gdn_real t4[NNO][3];
+			for (int v = 0; v < NNO; v++)
+				for (int d = 0; d < 3; d++)
+					t4[v][d] = m->node[3 * vid[v] + d];
+			gdn_real xr[3];
+			t10_get_ref_node_coord(i, xr);
+			gdn_tetra_ref2phy(t4, xr, NULL, 0, xq + 3 * (size_t)id[pos], NULL, NULL, NULL,
+					  NULL);
 		}
 	}
-	imap_free(&h);
+	free(first);
+	free(id);
 	free(m->node);
-	m->node = realloc(xq, 3 * (size_t)nq * sizeof(gdn_real));
-	if (!m->node)
-		die("out of memory");
+	m->node = xq;
 	m->nbqnodes_in = nq;
 }
 
 /* src/mesh/mesh.c:1122-1175 */
 static void build_fnode2enode(gdn_mesh *m)
 {
-	const int ne = m->nbelems;
-	m->fnode2enode = xcalloc(2 * NQV * (size_t)m->nbfaces, sizeof(int));
+	const int ne = m->nbelems, nf = m->nbfaces;
+	m->fnode2enode = xcalloc(2 * NQV * (size_t)nf, sizeof(int));
 	m->elem2qnode = realloc(m->elem2qnode,
 				((size_t)NQN * ne + (size_t)NQV * m->nboundaryfaces) *
 					sizeof(int));
 	if (!m->elem2qnode)
 		die("out of memory");
-	int bf = -1;
-	for (int f = 0; f < m->nbfaces; f++) {
+	/* boundary face number of each face (its place in boundaryface) */
+	int *bfn = xcalloc((size_t)(nf > 0 ? nf : 1), sizeof(int));
+	int nbf = 0;
+	for (int f = 0; f < nf; f++) {
+		bfn[f] = nbf;
+		nbf += m->face2elem[E2F * f + 2] < 0;
+	}
+	assert(nbf == m->nboundaryfaces);
+	int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nparts()) schedule(static) reduction(+ : bad)
+#endif
+	for (int f = 0; f < nf; f++) {
 		const int ieL = m->face2elem[E2F * f + 0];
 		const int ifL = m->face2elem[E2F * f + 1];
 		const int ieR = m->face2elem[E2F * f + 2];
-		if (ieR < 0)
-			bf++;
 		for (int a = 0; a < NQV; a++) {
 			const int iL = t10_get_ref_face2node_entry(ifL, a);
 			m->fnode2enode[2 * NQV * f + a] = iL;
 			const int q = m->elem2qnode[NQN * ieL + iL];
 			if (ieR < 0) {
 				m->fnode2enode[2 * NQV * f + NQV + a] = a;
-				m->elem2qnode[NQN * ne + NQV * bf + a] = q;
+				m->elem2qnode[NQN * ne + NQV * bfn[f] + a] = q;
 			} else {
 				int iR = 0;
 				while (iR < NQN && m->elem2qnode[NQN * ieR + iR] != q)
 					iR++;
-				if (iR == NQN)
-					die("face nodes of neighbouring elements do not match");
+				bad += iR == NQN;
 				m->fnode2enode[2 * NQV * f + NQV + a] = iR;
 			}
 		}
 	}
-	assert(bf + 1 == m->nboundaryfaces);
+	free(bfn);
+	if (bad)
+		die("face nodes of neighbouring elements do not match");
 }
 
 /* src/mesh/mesh.c:1178-1231: node2elem rows of width max_node2elem,
- * -1 terminated */
+ * -1 terminated; each thread fills the rows of its nodes (q mod np),
+ * scanning the elements in order */
 static void build_node2elem(gdn_mesh *m)
 {
 	const int nq = m->nbqnodes_in;
+	const size_t npos = (size_t)NQN * m->nbelems;
+	const int np = nparts();
 	int *count = xcalloc((size_t)nq, sizeof(int));
-	for (int i = 0; i < NQN * m->nbelems; i++)
-		count[m->elem2qnode[i]]++;
 	int mx = 0;
-	for (int q = 0; q < nq; q++)
-		mx = count[q] > mx ? count[q] : mx;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(np) reduction(max : mx)
+#endif
+	{
+		const int p = part_id();
+		for (size_t i = 0; i < npos; i++) {
+			const int q = m->elem2qnode[i];
+			if (q % np == p && ++count[q] > mx)
+				mx = count[q];
+		}
+	}
 	m->max_node2elem = mx + 1;
 	m->node2elem = malloc(((size_t)m->max_node2elem + 1) * nq * sizeof(int));
 	if (!m->node2elem)
@@ -605,11 +702,18 @@ static void build_node2elem(gdn_mesh *m)
 	for (size_t i = 0; i < (size_t)m->max_node2elem * nq; i++)
 		m->node2elem[i] = -1;
 	memset(count, 0, (size_t)nq * sizeof(int));
-	for (int e = 0; e < m->nbelems; e++)
-		for (int i = 0; i < NQN; i++) {
-			const int q = m->elem2qnode[NQN * e + i];
-			m->node2elem[(size_t)m->max_node2elem * q + count[q]++] = e;
-		}
+#ifdef _OPENMP
+#pragma omp parallel num_threads(np)
+#endif
+	{
+		const int p = part_id();
+		for (int e = 0; e < m->nbelems; e++)
+			for (int i = 0; i < NQN; i++) {
+				const int q = m->elem2qnode[NQN * e + i];
+				if (q % np == p)
+					m->node2elem[(size_t)m->max_node2elem * q + count[q]++] = e;
+			}
+	}
 	free(count);
 }
 

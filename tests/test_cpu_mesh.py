@@ -185,3 +185,33 @@ def test_msh_writer_round_trip(tmp_path):
     assert np.array_equal(np.ctypeslib.as_array(r.elem2node, shape=(4 * r.nbelems,)), e0)
     lib.gdn_gdn_mesh_free(C.byref(m))
     lib.gdn_gdn_mesh_free(C.byref(r))
+
+
+_CHECKSUM = r"""
+import sys, hashlib
+sys.path.insert(0, sys.argv[1])
+import gdon3d as G
+m = G.Mesh(kuhn=(16, 0.2, 7))
+h = hashlib.sha256()
+for k, v in sorted(m.arrays().items()):
+    h.update(k.encode()); h.update(v.tobytes())
+h.update(bytes(str((m.s.nbfaces, m.s.nboundaryfaces, m.s.nbqnodes_in, m.s.max_node2elem)), "ascii"))
+print(h.hexdigest())
+"""
+
+
+def test_connectivity_identical_for_any_thread_count():
+    """The connectivity is built on the host's threads (hash partitions per
+    thread, prefix-sum numbering): every array is the same bit for bit with
+    1, 3 and 8 threads (Kuhn 16^3 x 6, +-20 % jitter)."""
+    import subprocess
+    import sys
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "dg-cfl-less-maxwell-solver_amd")
+    out = set()
+    for t in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        r = subprocess.run([sys.executable, "-c", _CHECKSUM, pkg], env=env, check=True,
+                           capture_output=True, text=True)
+        out.add(r.stdout.strip().splitlines()[-1])
+    assert len(out) == 1
