@@ -95,6 +95,7 @@ struct gdh_ctx {
 	unsigned *queue; /* item counters of the persistent sweep (two slots) */
 	int qslot; /* the slot of the next launch */
 	int sweep_queue; /* 1: items taken from the counter, 0: static order */
+	int coop_refused; /* the runtime refused a cooperative launch */
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */

@@ -274,6 +274,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->queue = nullptr;
 	c->qslot = 0;
 	c->sweep_queue = 0;
+	c->coop_refused = 0;
 	c->cap_warned = 0;
 	c->nlagged = 0;
 	c->lag_last = c->lag_max = 0;

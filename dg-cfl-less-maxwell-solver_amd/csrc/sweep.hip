@@ -1540,9 +1540,39 @@ __global__ void k_epoch_bump(unsigned *e)
 	*e = n == 0u ? 2u : n; /* wrapped: keep the parity alternating */
 }
 
+/*
+ * A cooperative launch the runtime refuses (seen when another process holds
+ * the device): the sweeps go on with ordinary launches, the item sweep
+ * taking its items from the queue, which needs only QUEUE_SHARDS resident
+ * workgroups. Any other error stops as every HIP error does.
+ */
+static bool coop_refused(gdh_ctx *c, hipError_t e, const char *where)
+{
+	if (e == hipSuccess)
+		return false;
+	if (e != hipErrorIllegalState && e != hipErrorNotSupported &&
+	    e != hipErrorCooperativeLaunchTooLarge) {
+		fprintf(stderr, "gdon3d-hip: %s failed: %s\n", where, hipGetErrorString(e));
+		abort();
+	}
+	(void)hipGetLastError();
+	if (!c->coop_refused)
+		fprintf(stderr, "gdon3d-hip: the runtime refused a cooperative launch (%s); the "
+				"transport sweep continues as the item sweep over its queue\n",
+			hipGetErrorString(e));
+	c->coop_refused = 1;
+	return true;
+}
+
 template <typename R>
 static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 {
+	/* once a cooperative launch was refused: the item sweep over its queue */
+	if (c->coop_refused && c->sweep_mode != 0) {
+		c->sweep_mode = 1;
+		c->sweep_queue = 1;
+	}
+	const unsigned epoch0 = c->epoch;
 	/* a sweep that reads every item after one in which some items did not
 	 * publish (cluster sweep): zero the words and restart the epochs, so
 	 * that no word left from two sweeps back passes for this one */
@@ -1600,8 +1630,12 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 		}
 		void *cargs[] = { &chead, &cdesc, &ncl, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq,
 				  &ll, &epoch, &edev, &status, &disc, &kp };
-		HIPCHECK(hipLaunchCooperativeKernel((const void *)ck, dim3(cgrid), dim3(64), cargs,
-						    0, c->stream));
+		if (coop_refused(c, hipLaunchCooperativeKernel((const void *)ck, dim3(cgrid), dim3(64),
+							       cargs, 0, c->stream),
+				 "the cluster sweep's cooperative launch")) {
+			c->epoch = epoch0;
+			sweep_step_t<R>(c, fo, fn, xl, relax_in);
+		}
 		return;
 	}
 	const int grid = std::min(c->sweep_grid, nitems);
@@ -1623,8 +1657,12 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 	}
 	void *args[] = { &desc, &nitems, &fo_c, &fn, &xl, &dinv, &nv, &ne, &neq, &ll,
 			 &epoch, &edev, &status, &disc, &relax_in, &kp, &queue, &slot };
-	HIPCHECK(hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(64), args, 0,
-					    c->stream));
+	if (coop_refused(c, hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(64),
+						       args, 0, c->stream),
+			 "the item sweep's cooperative launch")) {
+		c->epoch = epoch0;
+		sweep_step_t<R>(c, fo, fn, xl, relax_in);
+	}
 }
 
 void sweep_step(gdh_ctx *c, void *fo, void *fn, const void *xl, int relax_in)

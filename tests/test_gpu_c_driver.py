@@ -5,7 +5,13 @@ simulation_set_time_parameters, dg_solve, simulation_error_l2,
 simulation_dump_info, simulation_free (src/utils/convergence.c:43-98) — on
 the reference's own mesh t4_cube_8.msh, RK3 and the relaxation scheme
 (D3Q4, D3Q6), against the oracle: the state after the solve to relative
-L2 <= 1e-12, the same step count, the L2 error to 1e-12 relative."""
+L2 <= 1e-12, the same step count, the L2 error to 1e-12 relative.
+
+The test process keeps a device context of its own while the driver runs:
+a second process on the device, where the runtime refuses cooperative
+launches (measured on MI355X: "the operation cannot be performed in the
+present state"); the library then runs the item sweep over its queue, and
+the driver's results are checked all the same."""
 import os
 import subprocess
 
@@ -26,12 +32,21 @@ K = 2 * np.pi * np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
                                              ("d3q6", 0.2, 3.0)])
 def test_c_driver_call_sequence_vs_oracle(scheme, tmax, cfl, tmp_path):
     assert os.path.exists(DRIVER), "build it first: make -C tests/c"
+    import gdon3d as G
+    G.set_cavity_mode(1, 1, 1, (1.0, -1.0, 0.0))
+    held = G.Simulation(G.Mesh(kuhn=(2, 0.0, 0)), G.Model("cavity_mode", kinetic="d3q4"))
+    held.set_time(0.1, 0.0, 5.0)
+    held_st = G.Stepper(held)  # this process's device context, alive during the run
     out = str(tmp_path / "wn.bin")
     print(f"C driver {scheme} tmax {tmax} cfl {cfl}", flush=True)
     r = subprocess.run([DRIVER, MESH, scheme, str(tmax), str(cfl), out], capture_output=True,
                        text=True, timeout=120)
     print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    held_st.free()
+    held.free()
     assert r.returncode == 0
+    print(f"  cooperative launch refused in the driver: {'refused a cooperative' in r.stderr}",
+          flush=True)
     itermax, err = r.stdout.strip().splitlines()[-1].split()
     wn = np.fromfile(out, dtype=np.float64)
     om = O.OMesh(MESH)
