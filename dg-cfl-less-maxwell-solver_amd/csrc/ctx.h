@@ -125,7 +125,7 @@ struct gdh_ctx {
 	int *cap_ipg, *cap_vel, *cap_off, *cap_piv;
 	double *cap_lu, *cap_t, *cap_part, *cap_rk;
 	long *cap_luoff;
-	/* matrix-free form (more lagged nodes than the dense LU takes): GMRES
+	/* matrix-free form (more lagged nodes than the dense inverse takes): GMRES
 	 * on the same systems, started from the last step's lagged values */
 	int cap_gmres;
 	std::vector<double> cap_zh, cap_krylov;

@@ -234,7 +234,10 @@ double gdh_timer_stop(gdh_ctx *ctx);
  * each cycle as one block, the GPU sweep lags the edges that close cycles
  * and solves exactly in capacitance form: x = sweep(f, z) with the lagged
  * values z from (I - T S) z = T sweep(f, 0), S the sweep's response to unit
- * lagged inputs (built and LU-factored once at set-up): two sweeps per step. */
+ * lagged inputs: up to 4096 lagged nodes per velocity (GDN_CAP_MAX_NODES)
+ * the matrix is built and inverted on the device at set-up and each step
+ * takes two sweeps; above, GMRES with sweeps as its products (2 + its
+ * products per step). */
 void gdh_transport_stats(gdh_ctx *ctx, int out[3]);
 
 /* Workgroups of the one-launch transport sweep: the number measured to be
