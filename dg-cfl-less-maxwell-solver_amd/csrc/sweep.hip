@@ -2433,12 +2433,14 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
  * relaxation kernel's pass over the state but gives every sweep item the
  * loads of all velocities at its nodes and the relaxation arithmetic, which
  * is hidden where the sweep is bound by its dependency chain and not where
- * it is bound by its item rate (measured on MI355X, D3Q4 FP64: 1.00 vs 1.03
- * ms per step at 32^3, 7.06 vs 6.57 ms at 64^3). The cluster sweep takes a
+ * it is bound by its item rate (measured on MI355X, D3Q4 item sweep per
+ * step, fused vs separate: FP64 32^3 0.883 vs 0.911 ms; FP32 32^3 0.875 vs
+ * 0.844 ms; 64^3 the cluster sweep wins). The cluster sweep takes a
  * relaxed state. Each candidate (item sweep with the relaxation separate or
  * fused, cluster sweep with it separate) is timed twice, five steps after
  * one each time, from the current state into the other buffer, and its
- * better time counts; nothing of the state changes. GDN_SWEEP and GDN_RELAX_FUSED=0|1 restrict the candidates.
+ * better time counts; nothing of the state changes. GDN_SWEEP and
+ * GDN_RELAX_FUSED=0|1 restrict the candidates.
  */
 static void relax_choose(gdh_ctx *c)
 {
