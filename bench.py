@@ -324,6 +324,8 @@ def bench_learned(args):
     t_train = time.perf_counter() - t0
     stats = G.learned_solve_stats(st, net, test_idx)
     ident = G.learned_solve_stats(st, None, test_idx)
+    gm = G.learned_solve_stats(st, net, test_idx, method="gmres")
+    gm_id = G.learned_solve_stats(st, None, test_idx, method="gmres")
     dof = 24 * 10 * nelem
     runs = {}
     for mode in ("exact", "learned"):
@@ -376,6 +378,10 @@ def bench_learned(args):
                    "richardson_fallback_rate": float(stats[2]),
                    "richardson_max_rel_l2_vs_exact": float(stats[3]),
                    "identity_preconditioner_fallback_rate": float(ident[2]),
+                   "gmres_mean_products": float(gm[0]),
+                   "gmres_max_products": float(gm[1]),
+                   "gmres_fallback_rate": float(gm[2]),
+                   "gmres_identity_mean_products": float(gm_id[0]),
                    "items_on_network_inverse": lr["stats"][0],
                    "items_on_stored_inverse": lr["stats"][1],
                    "refined_inverse_max_rel_deviation": lr["stats"][2],

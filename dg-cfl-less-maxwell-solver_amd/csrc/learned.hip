@@ -653,18 +653,123 @@ __global__ void k_learned_data(const int2 *__restrict__ items, const int *__rest
 
 /*
  * Block solves D u = r of the items idx[k] for 6 right-hand sides each
- * (uniform in [-1, 1], hashed from the item and the side): preconditioned
- * Richardson u <- u + P (r - D u), u_0 = P r, with P = I + pred (the
- * network's prediction, or the identity when pred is null), to
- * |r - D u| <= tol |r|; at most maxit iterations, then the exact solution
- * D^-1 r (fallback). Per side: iterations (or -1 for a fallback) and
+ * (uniform in [-1, 1], hashed from the item and the side) with P = I + pred
+ * (the network's prediction, or the identity when pred is null): method 0
+ * preconditioned Richardson u <- u + P (r - D u), u_0 = P r; method 1
+ * right-preconditioned GMRES (gmres_block); to |r - D u| <= tol |r| within
+ * maxit iterations (products with D), else the exact solution D^-1 r
+ * (fallback). Per side: iterations (or -1 for a fallback) and
  * |u - D^-1 r| / |D^-1 r|.
  */
+/* right-preconditioned GMRES on D P y = r, u = P y, restarted every
+ * LG_RESTART products (Saad and Schultz; modified Gram-Schmidt, Givens
+ * rotations), from u = 0 to |r - D u| <= tol |r| within maxit products;
+ * returns the products made, or -1 */
+#define LG_RESTART 10
+__device__ inline int gmres_block(const double (&D)[10][10], const double (&P)[10][10],
+				  const double (&r)[10], double rn2, double tol, int maxit,
+				  double (&u)[10])
+{
+	for (int i = 0; i < 10; i++)
+		u[i] = 0.0;
+	const double lim = tol * sqrt(rn2);
+	int total = 0;
+	while (true) {
+		double V[LG_RESTART + 1][10], H[LG_RESTART + 1][LG_RESTART], cs[LG_RESTART],
+			sn[LG_RESTART], g[LG_RESTART + 1];
+		double beta2 = 0.0;
+		for (int i = 0; i < 10; i++) {
+			double s = r[i];
+			for (int j = 0; j < 10; j++)
+				s -= D[i][j] * u[j];
+			V[0][i] = s;
+			beta2 += s * s;
+		}
+		const double beta = sqrt(beta2);
+		if (beta <= lim)
+			return total;
+		if (total >= maxit || !(beta == beta))
+			return -1;
+		for (int i = 0; i < 10; i++)
+			V[0][i] /= beta;
+		for (int i = 0; i <= LG_RESTART; i++)
+			g[i] = 0.0;
+		g[0] = beta;
+		int nc = 0;
+		for (int j = 0; j < LG_RESTART && total < maxit; j++) {
+			double z[10], w[10];
+			for (int i = 0; i < 10; i++) {
+				double s = 0.0;
+				for (int l = 0; l < 10; l++)
+					s += P[i][l] * V[j][l];
+				z[i] = s;
+			}
+			for (int i = 0; i < 10; i++) {
+				double s = 0.0;
+				for (int l = 0; l < 10; l++)
+					s += D[i][l] * z[l];
+				w[i] = s;
+			}
+			total++;
+			for (int q = 0; q <= j; q++) {
+				double h = 0.0;
+				for (int i = 0; i < 10; i++)
+					h += w[i] * V[q][i];
+				H[q][j] = h;
+				for (int i = 0; i < 10; i++)
+					w[i] -= h * V[q][i];
+			}
+			double hn = 0.0;
+			for (int i = 0; i < 10; i++)
+				hn += w[i] * w[i];
+			hn = sqrt(hn);
+			H[j + 1][j] = hn;
+			for (int i = 0; i < 10; i++)
+				V[j + 1][i] = hn > 0 ? w[i] / hn : 0.0;
+			for (int q = 0; q < j; q++) {
+				const double a0 = H[q][j], a1 = H[q + 1][j];
+				H[q][j] = cs[q] * a0 + sn[q] * a1;
+				H[q + 1][j] = -sn[q] * a0 + cs[q] * a1;
+			}
+			const double rho = sqrt(H[j][j] * H[j][j] + hn * hn);
+			cs[j] = rho > 0 ? H[j][j] / rho : 1.0;
+			sn[j] = rho > 0 ? hn / rho : 0.0;
+			H[j][j] = rho;
+			H[j + 1][j] = 0.0;
+			g[j + 1] = -sn[j] * g[j];
+			g[j] = cs[j] * g[j];
+			nc = j + 1;
+			if (fabs(g[j + 1]) <= lim || hn == 0.0)
+				break;
+		}
+		double y[LG_RESTART];
+		for (int q = nc - 1; q >= 0; q--) {
+			double s = g[q];
+			for (int l = q + 1; l < nc; l++)
+				s -= H[q][l] * y[l];
+			y[q] = s / H[q][q];
+		}
+		double x[10];
+		for (int i = 0; i < 10; i++) {
+			double s = 0.0;
+			for (int q = 0; q < nc; q++)
+				s += y[q] * V[q][i];
+			x[i] = s;
+		}
+		for (int i = 0; i < 10; i++) {
+			double s = 0.0;
+			for (int l = 0; l < 10; l++)
+				s += P[i][l] * x[l];
+			u[i] += s;
+		}
+	}
+}
+
 __global__ void k_learned_solve(const int2 *__restrict__ items, const int *__restrict__ idx,
 				int n, const double *__restrict__ geom,
 				const uint8_t *__restrict__ cls, KinPar kp, double dt,
 				const double *__restrict__ dinv, BatchTab bt,
-				const float *__restrict__ pred, double tol, int maxit,
+				const float *__restrict__ pred, double tol, int maxit, int method,
 				int *__restrict__ iters, double *__restrict__ relerr)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,7 +800,9 @@ __global__ void k_learned_solve(const int2 *__restrict__ items, const int *__res
 		u[i] = t;
 	}
 	int nit = -1;
-	for (int m = 1; m <= maxit; m++) {
+	if (method == 1)
+		nit = gmres_block(D, P, r, rn, tol, maxit, u);
+	for (int m = 1; method == 0 && m <= maxit; m++) {
 		double res[10], nr = 0.0;
 		for (int i = 0; i < 10; i++) {
 			double s = r[i];
@@ -756,14 +863,21 @@ extern "C" void gdh_learned_data(gdh_ctx *c, int n, const int *idx, float *x, fl
 
 /*
  * Block solves of the items idx with the network (mlp) or, when mlp is
- * null, the identity as preconditioner. out[0] mean Richardson iterations
+ * null, the identity as preconditioner, by Richardson (method 0) or GMRES
+ * (method 1). out[0] mean iterations
  * of the solves that converged, out[1] largest, out[2] fallback rate,
  * out[3] largest relative L2 difference to the exact solution, out[4]
  * solves. iters (6 n, may be null): per solve, -1 = fallback.
  */
-extern "C" void gdh_learned_solve_stats(gdh_ctx *c, gdh_mlp *mlp, int n, const int *idx,
-					double tol, int maxit, double *out, int *iters)
+extern "C" void gdh_learned_solve_stats_ex(gdh_ctx *c, gdh_mlp *mlp, int n, const int *idx,
+					   double tol, int maxit, int method, double *out,
+					   int *iters)
 {
+	if (method != 0 && method != 1) {
+		fprintf(stderr, "gdon3d-hip: block solve method %d (0 Richardson, 1 GMRES)\n",
+			method);
+		abort();
+	}
 	int *di = upload_idx(idx, n);
 	float *pred = nullptr;
 	if (mlp) {
@@ -778,7 +892,7 @@ extern "C" void gdh_learned_solve_stats(gdh_ctx *c, gdh_mlp *mlp, int n, const i
 	double *der = (double *)dmalloc(sizeof(double) * 6 * (size_t)n);
 	k_learned_solve<<<nblocks(6L * n, 64), 64, 0, c->stream>>>(
 		c->items, di, n, c->geom, c->cls, c->kp, c->dt, c->dinv, c->bt, pred, tol, maxit,
-		dit, der);
+		method, dit, der);
 	HIPCHECK(hipGetLastError());
 	std::vector<int> hit(6 * (size_t)n);
 	std::vector<double> her(6 * (size_t)n);
@@ -809,6 +923,12 @@ extern "C" void gdh_learned_solve_stats(gdh_ctx *c, gdh_mlp *mlp, int n, const i
 	for (void *q : { (void *)di, (void *)pred, (void *)dit, (void *)der })
 		if (q)
 			HIPCHECK(hipFree(q));
+}
+
+extern "C" void gdh_learned_solve_stats(gdh_ctx *c, gdh_mlp *mlp, int n, const int *idx,
+					double tol, int maxit, double *out, int *iters)
+{
+	gdh_learned_solve_stats_ex(c, mlp, n, idx, tol, maxit, 0, out, iters);
 }
 
 /* ------------------------------------------------------------------ */

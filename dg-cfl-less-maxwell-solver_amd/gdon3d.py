@@ -192,6 +192,9 @@ def load_library(path=LIB_PATH):
         "gdh_rk3_info": (None, [C.c_void_p, C.POINTER(C.c_int)]),
         "gdh_learned_solve_stats": (None, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                            C.c_double, C.c_int, c_double_p, C.POINTER(C.c_int)]),
+        "gdh_learned_solve_stats_ex": (None, [C.c_void_p, C.c_void_p, C.c_int,
+                                              C.POINTER(C.c_int), C.c_double, C.c_int, C.c_int,
+                                              c_double_p, C.POINTER(C.c_int)]),
         "gdh_learned_enable": (None, [C.c_void_p, C.c_void_p, C.c_double, c_double_p]),
         "t10_get_wpg_ref_entry": (C.c_double, [C.c_int]),
         "t10_get_mass_ref_entry": (C.c_double, [C.c_int, C.c_int]),
@@ -575,14 +578,16 @@ def learned_data(stepper, idx):
     return x, y
 
 
-def learned_solve_stats(stepper, net, idx, tol=1e-8, maxit=50):
-    """(mean iterations, max, fallback rate, max rel. L2 to exact, solves)"""
+def learned_solve_stats(stepper, net, idx, tol=1e-8, maxit=50, method="richardson"):
+    """(mean iterations, max, fallback rate, max rel. L2 to exact, solves);
+    method "richardson" or "gmres" (right-preconditioned, restart 10)"""
     lib = load_library()
     idx = np.ascontiguousarray(idx, np.int32)
     out = np.zeros(5)
     ctx = lib.dg_stepper_device_context(stepper.h)
-    lib.gdh_learned_solve_stats(ctx, net.h if net is not None else None, len(idx), iptr(idx),
-                                tol, maxit, dptr(out), None)
+    m = {"richardson": 0, "gmres": 1}[method]
+    lib.gdh_learned_solve_stats_ex(ctx, net.h if net is not None else None, len(idx),
+                                   iptr(idx), tol, maxit, m, dptr(out), None)
     return out
 
 

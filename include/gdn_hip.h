@@ -340,6 +340,11 @@ void gdh_learned_data(gdh_ctx *ctx, int n, const int *idx, float *x, float *y);
  * (or NULL) per solve, -1 = fallback */
 void gdh_learned_solve_stats(gdh_ctx *ctx, gdh_mlp *mlp, int n, const int *idx,
 			     double tol, int maxit, double *out, int *iters);
+/* the same by Richardson (method 0) or right-preconditioned GMRES (method
+ * 1, restarted every 10 products, from u = 0); iterations count products
+ * with D */
+void gdh_learned_solve_stats_ex(gdh_ctx *ctx, gdh_mlp *mlp, int n, const int *idx,
+				double tol, int maxit, int method, double *out, int *iters);
 
 /* Learned block inverses in the transport sweep: the network mlp predicts
  * D^-1 - I of every sweep item from its seven numbers; the predictions are

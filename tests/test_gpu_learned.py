@@ -10,7 +10,10 @@
   network reach |r - D u| <= 1e-8 |r| in about 7 iterations on average with
   at most 1 % fallbacks, where the identity preconditioner (always) and the
   untrained network (mostly) fall back to the exact inverse; every result is
-  within 1e-6 (relative L2) of the exact solution D^-1 r.
+  within 1e-6 (relative L2) of the exact solution D^-1 r. Right-preconditioned
+  GMRES converges on every solve within 10 products with any of the three
+  (measured: identity 6.7 on average, trained 6.9, untrained 10), so the
+  network pays only for Richardson.
 """
 import numpy as np
 import pytest
@@ -86,6 +89,16 @@ def test_training_and_what_the_network_gains():
     assert trained[2] <= 0.01 and trained[0] <= 12
     assert trained[3] <= 1e-6
     assert ident[2] > 0.5 and untrained[2] > trained[2] + 0.2
+    # right-preconditioned GMRES on the same solves: every one converges
+    # within 10 products whatever the preconditioner (a 10x10 system; the
+    # identity needs about 7 here, the trained network about as many), so
+    # the network's gain is Richardson's, not GMRES's
+    g = {name: G.learned_solve_stats(st2, p, idx, method="gmres")
+         for name, p in (("trained", net), ("identity", None), ("untrained", fresh))}
+    for name, st_ in g.items():
+        print(f"  GMRES {name}: mean products {st_[0]:.2f}, max {st_[1]:.0f}, fallback rate "
+              f"{st_[2]:.3f}, max rel L2 to exact {st_[3]:.2e}", flush=True)
+    assert all(st_[2] == 0.0 and st_[3] <= 1e-6 and st_[1] <= 10 for st_ in g.values())
     fresh.free()
     net.free()
     st2.free()
