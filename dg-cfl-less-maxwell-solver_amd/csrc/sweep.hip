@@ -1495,13 +1495,15 @@ void sweep_setup(gdh_ctx *c)
 	c->sweep_mode = !mode ? -1 : !strcmp(mode, "level") ? 0
 			: (!strcmp(mode, "item") || !strcmp(mode, "persistent")) ? 1
 			: !strcmp(mode, "cluster") ? 2 : -1;
-	/* waiting items poll one word per inflow face, then the traces
-	 * ("sentinel", the binary64 default), or the whole traces on every
-	 * pass ("full", the binary32 default: one word per value); measured
-	 * on 32^3 D3Q4: 0.910 vs 0.925 ms (binary64), 0.760 vs 0.738 ms
-	 * (binary32). GDN_SWEEP_POLL=sentinel|full overrides. */
+	/* waiting items poll the whole traces on every pass ("full", the
+	 * default), or one word per inflow face and then the traces
+	 * ("sentinel"); measured on MI355X, D3Q4 binary64 item sweep with one
+	 * hand-off word per value, full vs sentinel: 32^3 fused 0.876 vs
+	 * 0.882 ms, separate 0.768 vs 0.796 ms, 64^3 5.67 vs 5.72 ms (with
+	 * two words per binary64 value the sentinel had won, 0.910 vs 0.925);
+	 * binary32 0.738 vs 0.760 ms. GDN_SWEEP_POLL=sentinel|full overrides. */
 	const char *poll = getenv("GDN_SWEEP_POLL");
-	c->sweep_poll = poll ? !strcmp(poll, "full") : c->fp32;
+	c->sweep_poll = poll ? strcmp(poll, "sentinel") != 0 : 1;
 	c->sweep_grid = c->fp32 ? sweep_residency<float>(c) : sweep_residency<double>(c);
 	c->sweep_grid = grid_cap(c->sweep_grid);
 }
