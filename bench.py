@@ -48,9 +48,9 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
 SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep)
-    ("item", 1): (1731032.9 / 0.5 + 739387.3) * 1024 / 196608,
-    ("item", 0): (1035249.0 / 0.5 + 739563.3) * 1024 / 196608,
-    ("cluster", 0): (911590.6 / 0.5 + 645902.6) * 1024 / 196608,
+    ("item", 1): (1761233.1 / 0.5 + 739240.9) * 1024 / 196608,
+    ("item", 0): (1043138.1 / 0.5 + 739405.8) * 1024 / 196608,
+    ("cluster", 0): (911504.5 / 0.5 + 645913.5) * 1024 / 196608,
 }
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
