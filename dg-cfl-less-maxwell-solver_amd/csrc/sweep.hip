@@ -444,9 +444,11 @@ __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict
 		P.d1 = 0.0;
 		return;
 	}
+	/* read once per sweep: non-temporal, which leaves L2 to the hand-off
+	 * words (32^3 binary64 sweep 5 % faster in both relaxation forms) */
 	const double *D = dinv + 100L * it;
-	P.d0 = D[t];
-	P.d1 = t < 36 ? D[64 + t] : 0.0;
+	P.d0 = __builtin_nontemporal_load(D + t);
+	P.d1 = t < 36 ? __builtin_nontemporal_load(D + 64 + t) : 0.0;
 	P.pw = 0u;
 }
 
@@ -760,7 +762,9 @@ __device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], 
 			for (int b = 0; b < 6; b++)
 				sol -= B.G[s][i][b] * X[s][6 * iw + b];
 		const int v = uni(B.v), e = uni(B.e);
-		fn[((long)v * ne + e) * 60 + t] = (R)sol;
+		/* no consumer reads it during the sweep: non-temporal (separate
+		 * relaxation 2 % faster, fused unchanged) */
+		__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
 		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 		if (wl)
