@@ -104,9 +104,10 @@ template <typename F> static float time_launches(gdh_ctx *c, int reps, F f)
 }
 
 /*
- * Tile sizes by measurement on the batch: the RK3 stage with 64 elements
- * per block staged in LDS (one or two lanes per element), or 128 or 256
- * elements per block over the global state (first-stage launches from the
+ * Tile sizes by measurement on the batch: the RK3 stage with its state
+ * staged in LDS, two lanes per element (32 or 128 elements per block) or
+ * one (64 or 128), or with 128 or 256 elements per block
+ * over the global state (first-stage launches from the
  * current state into the other buffer, before the run starts), the
  * relaxation with 128, 256 or 512 nodes per block (on the other buffer).
  * The candidates are timed in two rounds and each keeps its faster round:
@@ -119,28 +120,31 @@ extern "C" void gdh_batch_autotune(gdh_ctx *c, int out[2])
 	const int prof = c->prof;
 	c->prof = 0;
 	if (!c->scheme) {
-		static const int cand[4][2] = { { 64, 2 }, { 64, 1 }, { 128, 0 }, { 256, 0 } };
+		/* { elements per block, kernel }: pair with one or four waves per
+		 * tile, tile with one or two, soa */
+		static const int cand[6][2] = { { 32, 2 }, { 128, 2 }, { 64, 1 }, { 128, 1 },
+						{ 128, 0 }, { 256, 0 } };
+		const int nc = 6;
 		/* GDN_RK3=pair|tile|soa keeps the candidates of that kernel */
 		const char *env = getenv("GDN_RK3");
 		const int only = !env ? -1 : !strcmp(env, "soa") ? 0 : !strcmp(env, "tile") ? 1
 				: !strcmp(env, "pair") ? 2 : -1;
-		float ms[4] = { 1e30f, 1e30f, 1e30f, 1e30f };
+		auto usable = [&](int k) { return only < 0 || cand[k][1] == only; };
+		float ms[6] = { 1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f };
 		for (int round = 0; round < 2; round++)
-			for (int k = 0; k < 4; k++) {
-				if (only >= 0 && cand[k][1] != only)
+			for (int k = 0; k < nc; k++) {
+				if (!usable(k))
 					continue;
 				c->rk3_bs = cand[k][0];
 				c->rk3_tile = cand[k][1];
 				ms[k] = std::min(ms[k], time_launches(c, 5, [&] { rk3_trial(c, 1); }));
 			}
 		int pick = -1;
-		for (int k = 0; k < 4; k++)
-			if ((only < 0 || cand[k][1] == only) && (pick < 0 || ms[k] < ms[pick]))
+		for (int k = 0; k < nc; k++)
+			if (usable(k) && (pick < 0 || ms[k] < ms[pick]))
 				pick = k;
 		c->rk3_bs = cand[pick][0];
 		c->rk3_tile = cand[pick][1];
-		if (c->rk3_tile == 2)
-			c->rk3_bs = 32; /* elements per block of the pair kernel */
 		/* the k2 stream with non-temporal or ordinary loads and stores,
 		 * timed on later-stage launches (the ones that read k2), unless
 		 * GDN_RK3_K2 fixes it */

@@ -243,7 +243,6 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 		c->bt_dmem[k] = nullptr;
 	}
 	c->graph[0] = c->graph[1] = nullptr;
-	c->rk3_bs = 128;
 	{
 		/* RK3 stage k2 stream: non-temporal (default) or ordinary loads and
 		 * stores; GDN_RK3_K2=nt|plain overrides (batches then skip timing it) */
@@ -264,6 +263,27 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 			c->rk3_tile = 1;
 		else if (k && !strcmp(k, "pair"))
 			c->rk3_tile = 2;
+		/* elements per block: the pair kernel stages 128 columns per block
+		 * of four waves (64^3 binary64 stage 1.072 ms; 1.099 with 64, 1.130
+		 * with 32), the tile kernel 128 per block of two (binary32 0.617 ms;
+		 * 0.629 with 64, 0.664 with 256: LDS then caps the waves per CU),
+		 * the global-state kernel 128 threads; GDN_RK3_BLOCK overrides with
+		 * a size the kernel has */
+		c->rk3_bs = 128;
+		const char *b = getenv("GDN_RK3_BLOCK");
+		if (b) {
+			const int v = atoi(b);
+			const bool ok = c->rk3_tile == 2 ? (v == 32 || v == 64 || v == 128)
+					: c->rk3_tile ? (v == 64 || v == 128 || (v == 256 && c->fp32))
+						      : (v == 64 || v == 128 || v == 256);
+			if (!ok) {
+				fprintf(stderr, "gdon3d-hip: GDN_RK3_BLOCK=%s is not a block size of "
+						"the %s kernel\n", b,
+					c->rk3_tile == 2 ? "pair" : c->rk3_tile ? "tile" : "soa");
+				abort();
+			}
+			c->rk3_bs = v;
+		}
 	}
 	c->relax_bs = 256;
 	c->cap_qtot = 0;

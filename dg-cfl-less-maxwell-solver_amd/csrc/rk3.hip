@@ -179,7 +179,7 @@ __device__ __forceinline__ void face_term(T (&acc)[6][10], const T *__restrict__
 
 /* face_term with the tile's values in LDS (Ws, column t) and the neighbour
  * trace from LDS when the neighbour column is in the tile [c0, c0 + 64) */
-template <int F, typename T>
+template <int F, typename T, int TL>
 __device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
 					       const T *__restrict__ W,
 					       const T *__restrict__ GH, long ne, long ngh,
@@ -201,10 +201,10 @@ __device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
 #pragma unroll
 	for (int b = 0; b < 6; b++) {
 		const int j = (map >> (4 * b)) & 15;
-		if (sizeof(T) == 4 && nt < 64u) {
+		if (sizeof(T) == 4 && nt < (unsigned)TL) {
 #pragma unroll
 			for (int q = 0; q < 6; q++)
-				wra[b][q] = Ws[(q * 10 + j) * 64 + nt];
+				wra[b][q] = Ws[(q * 10 + j) * TL + nt];
 		} else if (nb >= 0) {
 #pragma unroll
 			for (int q = 0; q < 6; q++)
@@ -223,7 +223,7 @@ __device__ __forceinline__ void face_term_tile(T (&acc)[6][10], const T *Ws,
 		T(&wr)[6] = wra[b];
 #pragma unroll
 		for (int q = 0; q < 6; q++)
-			wl[q] = Ws[(q * 10 + i) * 64 + t];
+			wl[q] = Ws[(q * 10 + i) * TL + t];
 		if (nb < 0 && bflux == GDH_FLUX_MAXWELL_METAL)
 			maxwell_metal(wl, vn, fl);
 		else
@@ -310,16 +310,17 @@ __global__ __launch_bounds__(BS) void k_rk3_soa(
 }
 
 /*
- * The same stage with the state of a tile of 64 consecutive storage columns
- * (a compact Morton block) staged in LDS by one coalesced pass: the
+ * The same stage with the state of a tile of 64 NW consecutive storage
+ * columns (a compact Morton block) staged in LDS by one coalesced pass: the
  * element's own values are read from LDS in every phase, and so is the
  * trace of a face neighbour inside the tile (about 70 % of the interior
- * faces at 64 elements per tile); neighbours outside the tile and ghosts
- * come from global memory. One 64-lane block per tile, one element per
- * lane; the arithmetic is that of k_rk3_soa.
+ * faces at 64 elements per tile, more in larger tiles); neighbours outside
+ * the tile and ghosts come from global memory. One block of NW 64-lane
+ * waves per tile, one element per lane; the arithmetic is that of
+ * k_rk3_soa.
  */
-template <typename T, bool NTK>
-__global__ __launch_bounds__(64) void k_rk3_tile(
+template <typename T, bool NTK, int NW>
+__global__ __launch_bounds__(64 * NW) void k_rk3_tile(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
 	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
@@ -327,10 +328,11 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 	double Bc, double dt, int first, int bflux, const int *__restrict__ inst,
 	const double *__restrict__ dt_tab, const double *__restrict__ ieps)
 {
-	__shared__ T Ws[60 * 64]; /* value (q, i) of tile column t at (10 q + i) 64 + t */
+	constexpr int TL = 64 * NW;
+	__shared__ T Ws[60 * TL]; /* value (q, i) of tile column t at (10 q + i) TL + t */
 	const long ne = ne_, ngh = ngh_;
 	const int tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int t = threadIdx.x, c0 = tile * 64, e = c0 + t;
+	const int t = threadIdx.x, c0 = tile * TL, e = c0 + t;
 	const bool live = e < ne_;
 	{
 		/* all 60 loads in flight, then the LDS stores */
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 			v[r] = live ? W[r * ne + e] : T(0);
 #pragma unroll
 		for (int r = 0; r < 60; r++)
-			Ws[r * 64 + t] = v[r];
+			Ws[r * TL + t] = v[r];
 	}
 	__syncthreads();
 	if (!live)
@@ -358,27 +360,27 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 		for (int m = 0; m < 10; m++)
 			acc[p][m] = T(0);
 
-	volume_field<0>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<0>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
-	volume_field<1>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<1>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
-	volume_field<2>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<2>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
-	volume_field<3>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<3>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
-	volume_field<4>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<4>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
-	volume_field<5>(acc, Ws, 64, t, ctd, Tb);
+	volume_field<5>(acc, Ws, TL, t, ctd, Tb);
 	pin(acc);
 
 	const int4 n4 = nbr[e], m4 = nbmap[e];
-	face_term_tile<0>(acc, Ws, W, GH, ne, ngh, t, c0, n4.x, m4.x, ctd, bflux, Tb);
+	face_term_tile<0, T, TL>(acc, Ws, W, GH, ne, ngh, t, c0, n4.x, m4.x, ctd, bflux, Tb);
 	pin(acc);
-	face_term_tile<1>(acc, Ws, W, GH, ne, ngh, t, c0, n4.y, m4.y, ctd, bflux, Tb);
+	face_term_tile<1, T, TL>(acc, Ws, W, GH, ne, ngh, t, c0, n4.y, m4.y, ctd, bflux, Tb);
 	pin(acc);
-	face_term_tile<2>(acc, Ws, W, GH, ne, ngh, t, c0, n4.z, m4.z, ctd, bflux, Tb);
+	face_term_tile<2, T, TL>(acc, Ws, W, GH, ne, ngh, t, c0, n4.z, m4.z, ctd, bflux, Tb);
 	pin(acc);
-	face_term_tile<3>(acc, Ws, W, GH, ne, ngh, t, c0, n4.w, m4.w, ctd, bflux, Tb);
+	face_term_tile<3, T, TL>(acc, Ws, W, GH, ne, ngh, t, c0, n4.w, m4.w, ctd, bflux, Tb);
 	pin(acc);
 
 	const T invdet = (T)(1.0 / det);
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(64) void k_rk3_tile(
 			const T d = acc[q][i] * (q < 3 ? idE : invdet);
 			const T k = first ? -dtT * d : AcT * k2_load<NTK>(K2 + g) - dtT * d;
 			k2_store<NTK>(k, K2 + g);
-			__builtin_nontemporal_store(Ws[(q * 10 + i) * 64 + t] + BcT * k, Wo + g);
+			__builtin_nontemporal_store(Ws[(q * 10 + i) * TL + t] + BcT * k, Wo + g);
 		}
 }
 
@@ -407,16 +409,18 @@ template <typename T> __device__ __forceinline__ void pin3(T (&acc)[3][10])
 }
 
 /*
- * Two lanes per element: a 64-lane block takes 32 consecutive storage
- * columns (15 KB of LDS in FP64, twice the blocks per CU of k_rk3_tile);
- * lane half h = 0 forms the E rows of the update, h = 1 the H rows. The
+ * Two lanes per element: each 64-lane wave takes 32 consecutive storage
+ * columns (15 KB of LDS in FP64), a block of NW waves a tile of 32 NW
+ * columns staged in LDS together (more face neighbours inside the tile:
+ * 64^3 binary64 stage 1.133 ms with 32 columns, 1.091 with 64, 1.076 with
+ * 128); lane half h = 0 forms the E rows of the update, h = 1 the H rows. The
  * code is the same for both halves: the curl terms of the H rows are those
  * of the E rows with E and H exchanged and the sign reversed (A(n): E-flux
  * = -n x H, H-flux = n x E), so a half differs only by its source fields
  * (H or E) and a sign sigma = -1 or +1. Both halves form the face fluxes;
  * each lifts its three components.
  */
-template <int J, typename T>
+template <int J, typename T, int TL>
 __device__ __forceinline__ void volume_pair(T (&acc)[3][10], const T *Ws, int c, int qs0,
 					    T sigma, const double (&ct)[9],
 					    const Rk3Tables<T> *__restrict__ Tb)
@@ -424,7 +428,7 @@ __device__ __forceinline__ void volume_pair(T (&acc)[3][10], const T *Ws, int c,
 	T wq[10];
 #pragma unroll
 	for (int j = 0; j < 10; j++)
-		wq[j] = Ws[((qs0 + J) * 10 + j) * 32 + c];
+		wq[j] = Ws[((qs0 + J) * 10 + j) * TL + c];
 	/* source component J feeds outputs o1, o2 with (sign * n_comp) */
 	constexpr int o1 = J == 0 ? 1 : 0, o2 = J == 2 ? 1 : 2;
 #pragma unroll
@@ -471,7 +475,7 @@ __device__ __forceinline__ void swap_halves(double &a, double &b)
 /* Neighbour trace of face F for the pair layout: lane half h gathers the
  * six fields at face nodes b = 3 h + b' (b' < 3), from the neighbour's
  * columns or the ghost DoFs of boundary face -1 - nb */
-template <int F, typename T>
+template <int F, typename T, int TL>
 __device__ __forceinline__ void gather_pair(T (&wra)[3][6], const T *Ws, const T *__restrict__ W,
 					    const T *__restrict__ GH, long ne, long ngh, int h,
 					    int c0, int nb, int map)
@@ -480,11 +484,11 @@ __device__ __forceinline__ void gather_pair(T (&wra)[3][6], const T *Ws, const T
 #pragma unroll
 	for (int bp = 0; bp < 3; bp++) {
 		const int b = 3 * h + bp;
-		if (nt < 32u) {
+		if (nt < (unsigned)TL) {
 			const int j = (map >> (4 * b)) & 15;
 #pragma unroll
 			for (int q = 0; q < 6; q++)
-				wra[bp][q] = Ws[(q * 10 + j) * 32 + nt];
+				wra[bp][q] = Ws[(q * 10 + j) * TL + nt];
 		} else if (nb >= 0) {
 			const int j = (map >> (4 * b)) & 15;
 #pragma unroll
@@ -506,7 +510,7 @@ __device__ __forceinline__ void gather_pair(T (&wra)[3][6], const T *Ws, const T
  * its E values of nodes 0-2 and takes its partner's of nodes 3-5, half 1
  * takes its partner's H values of nodes 0-2 and keeps its own of nodes
  * 3-5. */
-template <int F, typename T>
+template <int F, typename T, int TL>
 __device__ __forceinline__ void face_pair(T (&acc)[3][10], const T *Ws, const T (&wra)[3][6],
 					  int c, int h, int nb, const double (&ctd)[9], int bflux,
 					  const Rk3Tables<T> *__restrict__ Tb)
@@ -522,7 +526,7 @@ __device__ __forceinline__ void face_pair(T (&acc)[3][10], const T *Ws, const T 
 		T wl[6], fl[6];
 #pragma unroll
 		for (int q = 0; q < 6; q++)
-			wl[q] = Ws[(q * 10 + i) * 32 + c];
+			wl[q] = Ws[(q * 10 + i) * TL + c];
 		if (nb < 0 && bflux == GDH_FLUX_MAXWELL_METAL)
 			maxwell_metal(wl, vn, fl);
 		else
@@ -541,8 +545,8 @@ __device__ __forceinline__ void face_pair(T (&acc)[3][10], const T *Ws, const T 
 	}
 }
 
-template <typename T, bool NTK>
-__global__ __launch_bounds__(64) void k_rk3_pair(
+template <typename T, bool NTK, int NW>
+__global__ __launch_bounds__(64 * NW) void k_rk3_pair(
 	const T *__restrict__ W, T *__restrict__ Wo, T *__restrict__ K2,
 	const T *__restrict__ GH, const double *__restrict__ G,
 	const int4 *__restrict__ nbr, const int4 *__restrict__ nbmap,
@@ -550,10 +554,13 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 	double Bc, double dt, int first, int bflux, const int *__restrict__ inst,
 	const double *__restrict__ dt_tab, const double *__restrict__ ieps)
 {
-	__shared__ T Ws[60 * 32]; /* value (q, i) of tile column c at (10 q + i) 32 + c */
+	constexpr int TL = 32 * NW;
+	__shared__ T Ws[60 * TL]; /* value (q, i) of tile column c at (10 q + i) TL + c */
 	const long ne = ne_, ngh = ngh_;
 	const int tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int c = threadIdx.x % 32, h = threadIdx.x / 32, c0 = tile * 32, e = c0 + c;
+	/* wave w holds columns 32 w .. 32 w + 31, both halves of each */
+	const int l = threadIdx.x % 64;
+	const int c = (threadIdx.x / 64) * 32 + l % 32, h = l / 32, c0 = tile * TL, e = c0 + c;
 	const bool live = e < ne_;
 	{
 		/* each half stages 30 rows, all loads in flight */
@@ -563,7 +570,7 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 			v[r] = live ? W[(30 * h + r) * ne + e] : T(0);
 #pragma unroll
 		for (int r = 0; r < 30; r++)
-			Ws[(30 * h + r) * 32 + c] = v[r];
+			Ws[(30 * h + r) * TL + c] = v[r];
 	}
 	__syncthreads();
 	if (!live)
@@ -586,17 +593,17 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 	 * during the volume term, face f + 1's during face f's flux */
 	const int4 n4 = nbr[e], m4 = nbmap[e];
 	T ga[3][6], gb[3][6];
-	gather_pair<0>(ga, Ws, W, GH, ne, ngh, h, c0, n4.x, m4.x);
-	volume_pair<0>(acc, Ws, c, qs0, sigma, ctd, Tb);
-	volume_pair<1>(acc, Ws, c, qs0, sigma, ctd, Tb);
-	volume_pair<2>(acc, Ws, c, qs0, sigma, ctd, Tb);
-	gather_pair<1>(gb, Ws, W, GH, ne, ngh, h, c0, n4.y, m4.y);
-	face_pair<0>(acc, Ws, ga, c, h, n4.x, ctd, bflux, Tb);
-	gather_pair<2>(ga, Ws, W, GH, ne, ngh, h, c0, n4.z, m4.z);
-	face_pair<1>(acc, Ws, gb, c, h, n4.y, ctd, bflux, Tb);
-	gather_pair<3>(gb, Ws, W, GH, ne, ngh, h, c0, n4.w, m4.w);
-	face_pair<2>(acc, Ws, ga, c, h, n4.z, ctd, bflux, Tb);
-	face_pair<3>(acc, Ws, gb, c, h, n4.w, ctd, bflux, Tb);
+	gather_pair<0, T, TL>(ga, Ws, W, GH, ne, ngh, h, c0, n4.x, m4.x);
+	volume_pair<0, T, TL>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	volume_pair<1, T, TL>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	volume_pair<2, T, TL>(acc, Ws, c, qs0, sigma, ctd, Tb);
+	gather_pair<1, T, TL>(gb, Ws, W, GH, ne, ngh, h, c0, n4.y, m4.y);
+	face_pair<0, T, TL>(acc, Ws, ga, c, h, n4.x, ctd, bflux, Tb);
+	gather_pair<2, T, TL>(ga, Ws, W, GH, ne, ngh, h, c0, n4.z, m4.z);
+	face_pair<1, T, TL>(acc, Ws, gb, c, h, n4.y, ctd, bflux, Tb);
+	gather_pair<3, T, TL>(gb, Ws, W, GH, ne, ngh, h, c0, n4.w, m4.w);
+	face_pair<2, T, TL>(acc, Ws, ga, c, h, n4.z, ctd, bflux, Tb);
+	face_pair<3, T, TL>(acc, Ws, gb, c, h, n4.w, ctd, bflux, Tb);
 	const T invdet = (T)(1.0 / det);
 	const T sc = h == 0 && ieps ? (T)(ieps[e] / det) : invdet;
 	const T dtT = (T)(inst ? dt_tab[inst[e]] : dt), AcT = (T)Ac, BcT = (T)Bc;
@@ -609,7 +616,7 @@ __global__ __launch_bounds__(64) void k_rk3_pair(
 			const T d = acc[p][i] * sc;
 			const T k = first ? -dtT * d : AcT * k2_load<NTK>(K2 + g) - dtT * d;
 			k2_store<NTK>(k, K2 + g);
-			__builtin_nontemporal_store(Ws[r * 32 + c] + BcT * k, Wo + g);
+			__builtin_nontemporal_store(Ws[r * TL + c] + BcT * k, Wo + g);
 		}
 }
 
@@ -624,24 +631,52 @@ static void rk3_launch_bs(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double
 	HIPCHECK(hipGetLastError());
 }
 
+template <typename T, bool NTK, int NW>
+static void rk3_launch_pair(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+			    double dt, int first, const void *W, void *Wo)
+{
+	k_rk3_pair<T, NTK, NW><<<nblocks(c->ne, 32 * NW), 64 * NW, 0, c->stream>>>(
+		(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
+		c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
+		c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
+	HIPCHECK(hipGetLastError());
+}
+
+template <typename T, bool NTK, int NW>
+static void rk3_launch_tile(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
+			    double dt, int first, const void *W, void *Wo)
+{
+	k_rk3_tile<T, NTK, NW><<<nblocks(c->ne, 64 * NW), 64 * NW, 0, c->stream>>>(
+		(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
+		c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
+		c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
+	HIPCHECK(hipGetLastError());
+}
+
+/* rk3_bs = elements per block: 32, 64 or 128 (pair), 64, 128 or, in
+ * binary32, 256 (tile; 60 KB of LDS), 64, 128 or 256 (soa) */
 template <typename T, bool NTK>
 static void rk3_launch_nt(gdh_ctx *c, const Rk3Tables<T> *tab, double Ac, double Bc,
 			  double dt, int first, const void *W, void *Wo)
 {
 	if (c->rk3_tile == 2) {
-		k_rk3_pair<T, NTK><<<nblocks(c->ne, 32), 64, 0, c->stream>>>(
-			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
-			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
-			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
-		HIPCHECK(hipGetLastError());
+		switch (c->rk3_bs) {
+		case 128: rk3_launch_pair<T, NTK, 4>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+		case 64: rk3_launch_pair<T, NTK, 2>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+		default: rk3_launch_pair<T, NTK, 1>(c, tab, Ac, Bc, dt, first, W, Wo); break;
+		}
 		return;
 	}
 	if (c->rk3_tile) {
-		k_rk3_tile<T, NTK><<<nblocks(c->ne, 64), 64, 0, c->stream>>>(
-			(const T *)W, (T *)Wo, (T *)c->k2, (const T *)c->gh, c->geom_soa, c->nbr,
-			c->nbmap, tab, c->ne, 6 * c->nb, Ac, Bc, dt, first, c->md.bflux_kind,
-			c->bt.einst_col, c->bt.dt, c->bt.ieps_col);
-		HIPCHECK(hipGetLastError());
+		if constexpr (sizeof(T) == 4)
+			if (c->rk3_bs == 256) {
+				rk3_launch_tile<T, NTK, 4>(c, tab, Ac, Bc, dt, first, W, Wo);
+				return;
+			}
+		if (c->rk3_bs >= 128)
+			rk3_launch_tile<T, NTK, 2>(c, tab, Ac, Bc, dt, first, W, Wo);
+		else
+			rk3_launch_tile<T, NTK, 1>(c, tab, Ac, Bc, dt, first, W, Wo);
 		return;
 	}
 	switch (c->rk3_bs) {

@@ -142,6 +142,50 @@ def test_rk3_tiled_stage_equals_global_stage(precision, boundary, monkeypatch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_rk3_block_sizes_bit_identical(precision, monkeypatch):
+    """Each RK3 stage kernel at every block size it has (GDN_RK3_BLOCK:
+    elements per block; the LDS-tile kernels stage that many columns, the
+    global-state kernel takes that many threads) performs the same arithmetic
+    per element: only whether a face neighbour's values come from LDS or from
+    global memory changes, not the values. So every block size gives the same
+    state bit for bit after a whole solve (12^3 x 6 tets, jittered, upwind
+    boundary: tiles of 32 to 256 columns, neighbours inside and outside), and
+    the one against the oracle holds for all. The context reports the block
+    size it runs."""
+    path = kuhn_msh(12, 0.1, 9)
+    G.set_plane_wave((2 * np.pi, 0, 0), (0, 1, 0))
+    sizes = {"pair": (32, 64, 128), "tile": (64, 128) + ((256,) if precision == "fp32" else ()),
+             "soa": (64, 128, 256)}
+    for kern, blocks in sizes.items():
+        monkeypatch.setenv("GDN_RK3", kern)
+        out = {}
+        for bs in blocks:
+            monkeypatch.setenv("GDN_RK3_BLOCK", str(bs))
+            mesh = G.Mesh(path=path)
+            sim = G.Simulation(mesh, G.Model("plane_wave"))
+            sim.set_time(0.05, 0.0, 0.5)
+            st = G.Stepper(sim, precision=precision)
+            info = G.rk3_info(st)
+            assert info["kernel"] == kern and info["block"] == bs, info
+            st.advance(sim.s.itermax)
+            st.sync()
+            st.finish()
+            out[bs] = sim.wn()
+            st.free()
+            sim.free()
+        ref = out[blocks[0]]
+        for bs in blocks[1:]:
+            print(f"{kern} {precision}: block {bs} vs {blocks[0]}: max |diff| = "
+                  f"{np.abs(out[bs] - ref).max():.3e}", flush=True)
+            assert np.array_equal(out[bs], ref)
+    o = run_oracle(path, O.PLANE_WAVE, plane_wave_par(), 0.05, 0.5)
+    d = rel_l2(out[blocks[0]], o["wn"])
+    print(f"soa {precision} vs oracle: rel L2 {d:.3e}", flush=True)
+    assert d <= (TOL if precision == "fp64" else 2e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_rk3_k2_stream_hints_bit_identical(precision, monkeypatch):
     """The RK3 stage kernels load and store k2 with non-temporal hints
     (GDN_RK3_K2=nt, the default for one mesh) or ordinary accesses
