@@ -48,15 +48,15 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
 SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep)
-    ("item", 1): (1761233.1 / 0.5 + 739240.9) * 1024 / 196608,
-    ("item", 0): (1043138.1 / 0.5 + 739405.8) * 1024 / 196608,
-    ("cluster", 0): (911504.5 / 0.5 + 645913.5) * 1024 / 196608,
+    ("item", 1): (1756591.9 / 0.5 + 737773.6) * 1024 / 196608,
+    ("item", 0): (1043219.5 / 0.5 + 737907.9) * 1024 / 196608,
+    ("cluster", 0): (911650.2 / 0.5 + 645901.0) * 1024 / 196608,
 }
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
-    ("fp64", "k_rk3_pair"): (1127196.9 / 0.5 + 1474591.1) * 1024 / 1572864,
-    ("fp32", "k_rk3_tile"): (746906.2 / 0.5 + 737280.0) * 1024 / 1572864,
+    ("fp64", "k_rk3_pair"): (1112042.4 / 0.5 + 1474595.8) * 1024 / 1572864,
+    ("fp32", "k_rk3_tile"): (668114.9 / 0.5 + 737280.2) * 1024 / 1572864,
 }
 
 
