@@ -325,22 +325,32 @@ struct SweepTables {
 };
 
 /*
- * One prepared item: everything that does not depend on the new values of
- * its inflow neighbours. Slot s < nslot is the s-th interior, not lagged,
- * inflow face: its face index, coefficient c_f, neighbour and node map.
+ * One prepared item of the item and level sweeps: everything that does not
+ * depend on the new values of its inflow neighbours. Slot s < nslot is the
+ * s-th interior, not lagged, inflow face: its face index, coefficient c_f,
+ * neighbour and node map.
  */
 struct ItemBuf {
-	double G[3][10][7]; /* G_s = c_s D^-1 ML_s of slot s (rows padded: the
-			     * ten rows read at once fall in distinct banks) */
+	double D[10][11]; /* D^-1 (rows padded: the ten rows read at once fall
+			   * in distinct banks) */
+	double Z[60]; /* after the wait: 2 f - sum_f c_f ML_f u_f */
+	double cf[3];
 	int v, e, nslot, pad;
-	int nb[3], map[3];
-	int loc[3]; /* cluster sweep: source code of the slot's trace */
+	int nb[3], map[3], face[3];
 };
 
-/* Per-block scratch of one item's preparation: D^-1, and 60 values (one
- * given trace at a time, then Z); the finish uses none of it. */
+/* Per-block scratch of the item and level sweeps: one given trace at a time
+ * (and the learned refinement's eight numbers); with learned inverses also
+ * the refinement's X and D (200 values). */
+template <bool LB> struct ItemScratch {
+	double S[60];
+	double L[LB ? 200 : 1];
+};
+
+/* Per-block scratch of the cluster sweep's items: D^-1, and 60 values (one
+ * given trace at a time, then Z). */
 struct PrepScratch {
-	double D[10][11]; /* D^-1, rows padded like G */
+	double D[10][11]; /* D^-1, rows padded */
 	double S[60];
 };
 
@@ -368,12 +378,17 @@ template <typename R, int NV> struct PrepLoads {
  *       = D^-1 (2 f - sum_f c_f ML_f u_f) - f,   c_f = s_f dt/det,
  * since (dt/det) Minv (1-theta) F_ee = D - I for theta = 1/2, with
  * ML_f = Minv L_f S. Every item publishes w = (1-theta) f + theta sol, which
- * is u_f for its downstream neighbours. Before the wait: base = D^-1 (2 f -
- * sum_g c_g ML_g u_g) - f over the ghost and lagged inflow faces g
- * (prep_issue: the loads; prep_compute: LDS only). After it:
- *   sol = base - D^-1 sum_s c_s ML_s w_nb,s
- * over the interior inflow slots s (sweep_finish). Lanes: t = 10 iw + i
- * (field, node) for vectors, t = 6 q + b (field, face node) for traces.
+ * is u_f for its downstream neighbours. Before the wait: z0 = 2 f -
+ * sum_g c_g ML_g u_g over the ghost and lagged inflow faces g, and D^-1 into
+ * the item's buffer (prep_issue: the loads; prep_compute: LDS only). After
+ * it (sweep_finish):
+ *   sol = D^-1 (z0 - sum_s c_s ML_s w_nb,s) - f
+ * over the interior inflow slots s. This direct form does less work per item
+ * than forming D^-1 ML_s before the wait (the 32^3 sweep is bound by how
+ * fast the workgroups turn items over, not by the length of the dependency
+ * chain: 0.754 -> 0.722 ms with the relaxation separate, 0.873 -> 0.845 ms
+ * fused). Lanes: t = 10 iw + i (field, node) for vectors, t = 6 q + b
+ * (field, face node) for traces.
  */
 /* state index of the inflow trace value of lane t < 36 on face f */
 __device__ inline long trace_index(const ItemDesc &d, int f, int t, int ne, int neq, int nv)
@@ -457,9 +472,8 @@ __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict
  * seven numbers, X0 = I + the network's prediction, then nsteps
  * Newton-Schulz steps R = I - D X, X <- X + X R, every entry a sum over
  * l = 0..9 in order (the set-up check does the same arithmetic). Lane t
- * owns entries t and (t < 36) 64 + t. X and D live in scr (200 doubles:
- * the G buffer of the item being prepared, written only afterwards), the
- * numbers in xs (8 doubles).
+ * owns entries t and (t < 36) 64 + t. X and D live in scr (200 doubles of
+ * the block's scratch), the numbers in xs (8 doubles).
  */
 template <typename R, int NV>
 __device__ inline void learned_inverse(const PrepLoads<R, NV> &P, double (*Dinv)[11],
@@ -523,24 +537,25 @@ __device__ inline void learned_inverse(const PrepLoads<R, NV> &P, double (*Dinv)
  * with x_nb from xl (none when xl is null: both halves of a lagged trace
  * are given values). With relax_in the item's values are relaxed first (the
  * state holds the previous transport's output) and rounded to R as a stored
- * state would be. Fills B (G_s and the slots) and fv (the lane's value of
- * the item, t < 60) and returns the lane's base value. */
+ * state would be. Fills B (D^-1 and the slots) and fv (the lane's value of
+ * the item, t < 60) and returns the lane's z0 = 2 f - sum over the given
+ * faces. */
 template <typename R, int NV, bool LB = false>
 __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> &P,
 				      const R *__restrict__ fo, const R *__restrict__ xl,
 				      int ne, int neq, int nv, int relax_in, const KinPar &kp,
-				      const SweepTables &T, ItemBuf &B, PrepScratch &W, R &fv)
+				      const SweepTables &T, ItemBuf &B, ItemScratch<LB> &W, R &fv)
 {
 	const int t = threadIdx.x;
 	fv = relax_in ? (R)relax_value<NV>(kp, P.fr, d.v, kp.ieps ? kp.ieps[d.e] : 1.0) : P.fr[0];
 	const int gf = given_faces(d), sf = slot_faces(d);
 	const int iw = t / 10, i = t % 10;
 	if (LB && (d.flags & LB_FLAG)) {
-		learned_inverse(P, W.D, &B.G[0][0][0], W.S, ((unsigned)d.flags >> 30) + 2);
+		learned_inverse(P, B.D, W.L, W.S, ((unsigned)d.flags >> 30) + 2);
 	} else {
-		W.D[t / 10][t % 10] = P.d0;
+		B.D[t / 10][t % 10] = P.d0;
 		if (t < 36)
-			W.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
+			B.D[(64 + t) / 10][(64 + t) % 10] = P.d1;
 	}
 	if (t == 0) {
 		int s = 0;
@@ -551,7 +566,8 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 				continue;
 			B.nb[s] = d.nb[f];
 			B.map[s] = d.map[f];
-			B.loc[s] = (d.flags >> (12 + 4 * f)) & 15;
+			B.face[s] = f;
+			B.cf[s] = d.cf[f];
 			s++;
 		}
 		B.nslot = s;
@@ -581,33 +597,8 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 		}
 		__syncthreads();
 	}
-	if (t < 60)
-		W.S[t] = 2.0 * (double)fv - y; /* Z[iw][i] */
 	__syncthreads();
-	double base = 0.0;
-	if (t < 60) {
-		double s = 0.0;
-#pragma unroll
-		for (int l = 0; l < 10; l++)
-			s += W.D[i][l] * W.S[10 * iw + l];
-		base = s - (double)fv;
-		/* G_s, lane t = 6 r + b: row r, face node b */
-		const int r = t / 6, b = t % 6;
-		int slot = 0;
-#pragma unroll
-		for (int f = 0; f < 4; f++) {
-			if (!(sf & (1 << f)))
-				continue;
-			double g = 0.0;
-#pragma unroll
-			for (int l = 0; l < 10; l++)
-				g += W.D[r][l] * T.ML[f][l][b];
-			B.G[slot][r][b] = d.cf[f] * g;
-			slot++;
-		}
-	}
-	__syncthreads();
-	return base;
+	return 2.0 * (double)fv - y;
 }
 
 __device__ inline u64 ll_load(const u64 *p)
@@ -743,32 +734,43 @@ __device__ inline bool spin_expired(int spins, int *status)
 	return false;
 }
 
-/* sol = base - sum_s G_s X_s with X[s] the published trace w of slot s's
- * neighbour; sol goes to fn and w = (1 - theta) fv + theta sol (fv, base:
- * the lane's old and base values) to the hand-off words, each rounded once
- * to R, and (cluster sweep) the rounded w to wl[t]. */
+/* sol = D^-1 (z0 - sum_s c_s ML_s X_s) - fv with X[s] the published trace w
+ * of slot s's neighbour (fv, z0: the lane's old value and z0); sol goes to
+ * fn and w = (1 - theta) fv + theta sol to the hand-off words, each rounded
+ * once to R. */
 template <typename R>
-__device__ inline void sweep_finish(const ItemBuf &B, const double (&X)[3][36], R fv,
-				    double base, R *__restrict__ fn, u64 *ll, int ne,
-				    unsigned epoch, double *wl = nullptr)
+__device__ inline void sweep_finish(ItemBuf &B, const double (&X)[3][36], R fv, double z0,
+				    R *__restrict__ fn, u64 *ll, int ne, unsigned epoch,
+				    const SweepTables &T)
 {
 	const int t = lane_id();
 	const int iw = t / 10, i = t % 10;
 	const int nslot = uni(B.nslot);
 	if (t < 60) {
-		double sol = base;
-		for (int s = 0; s < nslot; s++)
+		double y = 0.0;
+		for (int s = 0; s < nslot; s++) {
+			const int f = uni(B.face[s]);
+			double sm = 0.0;
 #pragma unroll
 			for (int b = 0; b < 6; b++)
-				sol -= B.G[s][i][b] * X[s][6 * iw + b];
+				sm += T.ML[f][i][b] * X[s][6 * iw + b];
+			y += B.cf[s] * sm;
+		}
+		B.Z[t] = z0 - y;
+	}
+	__syncthreads();
+	if (t < 60) {
+		double sm = 0.0;
+#pragma unroll
+		for (int l = 0; l < 10; l++)
+			sm += B.D[i][l] * B.Z[10 * iw + l];
+		const double sol = sm - (double)fv;
 		const int v = uni(B.v), e = uni(B.e);
 		/* no consumer reads it during the sweep: non-temporal (separate
 		 * relaxation 2 % faster, fused unchanged) */
 		__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
 		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
-		if (wl)
-			wl[t] = (double)published(wr, epoch);
 	}
 	__syncthreads();
 }
@@ -792,7 +794,7 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B;
-	__shared__ PrepScratch W;
+	__shared__ ItemScratch<LB> W;
 	__shared__ double X[3][36];
 	load_tables(T);
 	const unsigned epoch = epoch_dev ? *epoch_dev : epoch_arg;
@@ -801,7 +803,7 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	PrepLoads<R, NV> P;
 	prep_issue<R, NV, LB>(it, d, fo, dinv, ne, relax_in, kp, P);
 	R fv;
-	const double base =
+	const double z0 =
 		prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B, W, fv);
 	const int t = threadIdx.x;
 	const unsigned pending =
@@ -809,7 +811,7 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 	if (__ballot(pending != 0) && t == 0)
 		__hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	__syncthreads();
-	sweep_finish<R>(B, X, fv, base, fn, ll, ne, epoch);
+	sweep_finish<R>(B, X, fv, z0, fn, ll, ne, epoch, T);
 }
 
 /* an ItemDesc held one dword per lane (lane k: dword k), made uniform */
@@ -878,7 +880,7 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 {
 	__shared__ SweepTables T;
 	__shared__ ItemBuf B[2];
-	__shared__ PrepScratch W;
+	__shared__ ItemScratch<LB> W;
 	__shared__ double X[3][36];
 	const int t = threadIdx.x;
 	if (disc) {
@@ -917,14 +919,14 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 	int k = 0;
 	if (it >= nitems)
 		return;
-	R fv; /* the lane's old and base values of the current item */
-	double base;
+	R fv; /* the lane's old value and z0 of the current item */
+	double z0;
 	{
 		const ItemDesc d = desc[it];
 		PrepLoads<R, NV> P;
 		prep_issue<R, NV, LB>(it, d, fo, dinv, ne, relax_in, kp, P);
-		base = prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W,
-					       fv);
+		z0 = prep_compute<R, NV, LB>(d, P, fo, xl, ne, neq, nv, relax_in, kp, T, B[0], W,
+					     fv);
 	}
 	ItemDesc dn; /* descriptor of the next item */
 	if (next < nitems)
@@ -941,15 +943,15 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 		if (nn < nitems && tl < (int)(sizeof(ItemDesc) / 4))
 			dnn = reinterpret_cast<const int *>(desc + nn)[tl];
 		const unsigned nnn_req = QU && nn < nitems ? queue_take(q) : 0u;
-		double nbase = 0.0;
+		double nz0 = 0.0;
 		R nfv = (R)0;
 		unsigned pending = t < 36 ? (1u << uni(B[k].nslot)) - 1u : 0u;
 		pending = ll_poll<R>(B[k], ll, ne, epoch, pending, X);
 		if (__ballot(pending != 0)) {
 			/* prepare the next item, then watch one word per slot */
 			if (has_next)
-				nbase = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
-								kp, T, B[k ^ 1], W, nfv);
+				nz0 = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
+							      kp, T, B[k ^ 1], W, nfv);
 			int spins = 0;
 			if (!FULL_POLL) {
 				bool ready = ll_sentinels<R>(B[k], ll, ne, epoch);
@@ -967,17 +969,17 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 					break;
 			}
 			__syncthreads(); /* X complete */
-			sweep_finish<R>(B[k], X, fv, base, fn, ll, ne, epoch);
+			sweep_finish<R>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
 		} else {
 			__syncthreads(); /* X complete */
-			sweep_finish<R>(B[k], X, fv, base, fn, ll, ne, epoch);
+			sweep_finish<R>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
 			if (has_next)
-				nbase = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
-								kp, T, B[k ^ 1], W, nfv);
+				nz0 = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
+							      kp, T, B[k ^ 1], W, nfv);
 		}
 		if (has_next) {
 			fv = nfv;
-			base = nbase;
+			z0 = nz0;
 		}
 		const int nnn = !QU ? nn + G
 				   : nn < nitems
