@@ -11,16 +11,17 @@ node-local relaxation.
 
 value = DoF-updates/s = (24 * 10 * nelem) * K / (wall time of K steps), with
 the state resident in HBM (no host transfer inside the timed region).
-roofline: the transport sweep (dominant kernel, one launch per step).
-Algorithmic bytes per DoF-update (s = 8 FP64, 4 FP32): transport reads and
-writes the 24 micro-fields (2 s), the relaxation reads and writes them (2 s),
-plus 112 B geometry + 16 B level indices per element (128 / 240 per DoF):
-4 s + 0.53 = 32.5 B (FP64), 16.5 B (FP32). When the relaxation is fused into
-the sweep, one sweep launch does all of it, (960 s + 128) B per element; when
-it runs as a pass of its own, the sweep launch does the transport half,
-(480 s + 128) B per element. achieved = those bytes per launch / the sweep's
-mean launch time (HIP events on the solver stream), against 8 TB/s;
-achieved_step = 32.5 B (16.5 B) x DoF-updates/s over the whole step.
+roofline: the kernels of one step, the transport sweep (one launch) and,
+when it runs as a kernel of its own, the relaxation pass. Algorithmic bytes
+per DoF-update (s = 8 FP64, 4 FP32): transport reads and writes the 24
+micro-fields (2 s), the relaxation reads and writes them (2 s), plus 112 B
+geometry + 16 B level indices per element (128 / 240 per DoF): 4 s + 0.53 =
+32.5 B (FP64), 16.5 B (FP32), (960 s + 128) B per element and step.
+achieved = those bytes / the mean device time of the step's kernels (HIP
+events on the solver stream: the sweep with the relaxation fused into it, or
+the sweep plus the pass), against 8 TB/s; achieved_step = the same bytes x
+DoF-updates/s over the whole step (ghosts and launch gaps included);
+achieved_transport_only = the transport half over the sweep kernel alone.
 cpu_baseline: the oracle's restatement of the reference CPU algorithm
 (assembled operators, sparse LU per velocity) on a bounded sample of the same
 workload: the first steps at 32^3, assembly excluded.
@@ -48,10 +49,12 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
 SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep)
-    ("item", 1): (1756591.9 / 0.5 + 737773.6) * 1024 / 196608,
-    ("item", 0): (1043219.5 / 0.5 + 737907.9) * 1024 / 196608,
-    ("cluster", 0): (911650.2 / 0.5 + 645901.0) * 1024 / 196608,
+    ("item", 1): (1745173.4 / 0.5 + 737745.2) * 1024 / 196608,
+    ("item", 0): (1035711.3 / 0.5 + 737874.2) * 1024 / 196608,
+    ("cluster", 0): (911510.9 / 0.5 + 645892.2) * 1024 / 196608,
 }
+# the relaxation pass of one step (k_relax<4, double>) at 32^3 D3Q4
+RELAX_TRAFFIC_BYTES_PER_ELEM = (184347.2 / 0.5 + 368686.2) * 1024 / 196608
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
@@ -486,6 +489,10 @@ def main():
     sim_itermax = sim.s.itermax
     default_workload = (n == 32 and not args.jitter and args.exact == "cavity"
                         and args.precision == "fp64" and not args.learned_inverses)
+    # measured HBM bytes of one step's kernels (the headline workload only)
+    step_traffic = None
+    if default_workload and traffic:
+        step_traffic = traffic + (0.0 if relax_fused else RELAX_TRAFFIC_BYTES_PER_ELEM)
     print(f"[bench] n={n} elements={nelem} itermax={sim.s.itermax} dt={sim.s.dt:.6e} "
           f"levels={st.levels} sweep={sweep} "
           f"relax_fused={relax_fused} setup={t_setup:.2f}s",
@@ -510,12 +517,17 @@ def main():
     relax_s = prof[1] / args.profile_steps
     launches = prof[3] / args.profile_steps
     s_val = 4 if args.precision == "fp32" else 8
-    # algorithmic bytes of one sweep launch: transport and relaxation when the
-    # relaxation is fused into the sweep, the transport half otherwise
+    # algorithmic bytes of one step per element: the transport and the
+    # relaxation each read and write the 24 micro-fields, plus geometry and
+    # schedule (32.5 B per DoF-update in binary64, 16.5 B in binary32),
+    # over the device time of that step's kernels: the sweep, and the
+    # relaxation pass when it runs as a kernel of its own (the same figure
+    # whichever form the set-up chose)
     transport_bytes = 480 * s_val + 112 + 16
-    sweep_bytes = 960 * s_val + 128 if relax_fused else transport_bytes
+    step_bytes = 960 * s_val + 128
     launch_s = sweep_s / max(launches, 1)
-    achieved = sweep_bytes * nelem / max(launches, 1) / launch_s / 1e9
+    kernels_s = sweep_s + relax_s
+    achieved = step_bytes * nelem / kernels_s / 1e9
     print(f"[bench] per step: sweep {sweep_s*1e3:.3f} ms ({launches:.0f} launches), "
           f"relax {relax_s*1e3:.3f} ms, ghosts {prof[2]/args.profile_steps*1e3:.3f} ms",
           file=sys.stderr, flush=True)
@@ -562,7 +574,7 @@ def main():
             "itermax": sim_itermax,
             "t_final": t_final,
             "l2_error": err,
-            "hbm_gb_s_sweep": achieved,
+            "hbm_gb_s": achieved,
             "sweep_ms": sweep_s * 1e3,
             "relaxation_pass_ms": relax_s * 1e3,
             "relaxation_fused_into_sweep": bool(relax_fused),
@@ -583,15 +595,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (traffic * nelem if default_workload and traffic else None),
-            "traffic_gb_s": (traffic * nelem / sweep_s / 1e9
-                             if default_workload and traffic else None),
-            "kernel": f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step"
-                      f"{', relaxation fused' if relax_fused else ''})",
-            "bytes_per_element_launch": sweep_bytes,
-            "bytes_per_dof_update": sweep_bytes / 240,
+            "traffic": step_traffic * nelem if step_traffic else None,
+            "traffic_gb_s": (step_traffic * nelem / kernels_s / 1e9 if step_traffic else None),
+            "kernel": (f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step, "
+                       f"relaxation fused)" if relax_fused else
+                       f"{SWEEP_KERNEL[sweep['form']]} + k_relax (the sweep and the relaxation "
+                       f"pass of one step)"),
+            "bytes_per_element_step": step_bytes,
+            "bytes_per_dof_update": step_bytes / 240,
             "launches_per_step": launches,
             "avg_launch_us": launch_s * 1e6,
+            "relaxation_pass_us": relax_s * 1e6,
+            "step_kernels_us": kernels_s * 1e6,
             "achieved_step": value * b_alg / 1e9,
             "frac_step": value * b_alg / 1e9 / HBM_PEAK_GBS,
             "bytes_per_dof_update_step": b_alg,
