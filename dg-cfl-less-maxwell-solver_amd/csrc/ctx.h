@@ -109,6 +109,8 @@ struct gdh_ctx {
 	std::vector<int> bt_e0, bt_nb0; /* first element / boundary face per instance */
 	hipGraphExec_t graph[2];
 	int rk3_bs, relax_bs; /* tile sizes (autotuned for batches) */
+	int relax_pair; /* relaxation pass: two elements per wave (k_relax_pair)
+			 * or one (k_relax) */
 	int rk3_tile; /* RK3 stage: 0 global state, 1 LDS tile, 2 LDS tile with
 		       * two lanes per element */
 	int rk3_ntk; /* RK3 stage: k2 with non-temporal loads and stores (1, the

@@ -380,3 +380,61 @@ def test_item_queue_equals_static_order(kin, n, jit, fused, monkeypatch):
     print(f"queue {kin} n={n} fused {fused}: ran {out['dynamic']['form']}", flush=True)
     assert out["static"]["form"] == out["dynamic"]["form"] == ("item", fused)
     assert np.array_equal(out["static"]["wn"], out["dynamic"]["wn"])
+
+
+def _kuhn_minus_last_tet(n, jitter, seed):
+    """A Kuhn n^3 x 6 mesh file without its last tetrahedron: an odd number
+    of elements (the remaining cells stay valid; the removed cell's faces
+    become boundary faces)."""
+    src = kuhn_msh(n, jitter, seed)
+    path = src.replace(".msh", "_odd.msh")
+    if not os.path.exists(path):
+        lines = open(src).read().split("\n")
+        i = lines.index("$Elements")
+        count = int(lines[i + 1])
+        tets = [k for k in range(i + 2, i + 2 + count) if lines[k].split()[1] == "4"]
+        del lines[tets[-1]]
+        lines[i + 1] = str(count - 1)
+        with open(path, "w") as f:
+            f.write("\n".join(lines))
+    return path
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kin,precision", [("d3q4", "fp64"), ("d3q4", "fp32"), ("d3q6", "fp64"),
+                                           ("d3q6", "fp32")])
+def test_relaxation_pass_pair_equals_wave(kin, precision, monkeypatch):
+    """The relaxation pass with two elements per wave and one 16-byte
+    (binary64) or 8-byte (binary32) access per lane and velocity
+    (GDN_RELAX_PASS=pair, the default) performs per value the arithmetic of
+    the pass with one element per wave (GDN_RELAX_PASS=wave): whole solves
+    with the pass after every sweep agree bit for bit, on a mesh with an odd
+    number of elements (the last wave holds one) and permittivity per
+    element (the flux on D / eps)."""
+    path = _kuhn_minus_last_tet(5, 0.1, 7)
+    mesh = G.Mesh(path=path)
+    ne = mesh.nelem
+    mesh.free()
+    assert ne % 2 == 1
+    eps = np.random.default_rng(5).uniform(1.0, 4.0, ne)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    monkeypatch.setenv("GDN_SWEEP", "item")
+    monkeypatch.setenv("GDN_RELAX_FUSED", "0")
+    out = {}
+    for kind in ("wave", "pair"):
+        monkeypatch.setenv("GDN_RELAX_PASS", kind)
+        mesh = G.Mesh(path=path)
+        sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic=kin))
+        sim.set_time(0.3, 0.0, 5.0)
+        st = G.Stepper(sim, precision=precision, eps=eps)
+        assert G.relax_fused(st) == 0
+        st.advance(sim.s.itermax)
+        st.sync()
+        st.finish()
+        out[kind] = sim.wn()
+        st.free()
+        sim.free()
+    print(f"relaxation pass pair vs wave {kin} {precision} ({ne} elements): max |diff| "
+          f"{np.abs(out['pair'] - out['wave']).max():.3e}", flush=True)
+    assert np.all(np.isfinite(out["pair"]))
+    assert np.array_equal(out["pair"], out["wave"])
