@@ -57,6 +57,10 @@ SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep
 RELAX_TRAFFIC_BYTES_PER_ELEM = (184347.2 / 0.5 + 368686.2) * 1024 / 196608
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
+# gdh_relax_fused: where the relaxation of a step is applied
+RELAX_FORMS = {0: "in a pass of its own after the sweep",
+               1: "fused into the next sweep's reads",
+               2: "by each element's last item inside the sweep"}
 RK3_TRAFFIC_BYTES_PER_ELEM_STAGE = {  # (precision, stage kernel)
     ("fp64", "k_rk3_pair"): (1112042.4 / 0.5 + 1474595.8) * 1024 / 1572864,
     ("fp32", "k_rk3_tile"): (668114.9 / 0.5 + 737280.2) * 1024 / 1572864,
@@ -577,7 +581,8 @@ def main():
             "hbm_gb_s": achieved,
             "sweep_ms": sweep_s * 1e3,
             "relaxation_pass_ms": relax_s * 1e3,
-            "relaxation_fused_into_sweep": bool(relax_fused),
+            "relaxation_fused_into_sweep": relax_fused == 1,
+            "relaxation_form": RELAX_FORMS.get(relax_fused, str(relax_fused)),
             "sweep_form": sweep["form"],
             "sweep_clusters": sweep["clusters"],
             "sweep_resident_workgroups": sweep["grid"],
@@ -598,7 +603,7 @@ def main():
             "traffic": step_traffic * nelem if step_traffic else None,
             "traffic_gb_s": (step_traffic * nelem / kernels_s / 1e9 if step_traffic else None),
             "kernel": (f"{SWEEP_KERNEL[sweep['form']]} (one launch = the whole sweep of one step, "
-                       f"relaxation fused)" if relax_fused else
+                       f"relaxation {RELAX_FORMS[relax_fused]})" if relax_fused else
                        f"{SWEEP_KERNEL[sweep['form']]} + k_relax (the sweep and the relaxation "
                        f"pass of one step)"),
             "bytes_per_element_step": step_bytes,

@@ -221,7 +221,7 @@ extern "C" void gdh_batch_advance(gdh_ctx *c, int nsteps)
 	/* kinetic: a captured step relaxes its input (the state awaits the
 	 * relaxation of the step before); a state that does not (fresh, or
 	 * materialized by a download) takes one direct step first */
-	if (c->scheme && c->relax_fused && !c->relax_pending && nsteps > 0) {
+	if (c->scheme && c->relax_fused == 1 && !c->relax_pending && nsteps > 0) {
 		batch_step(c);
 		nsteps--;
 	}

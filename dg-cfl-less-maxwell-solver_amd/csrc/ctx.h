@@ -99,6 +99,8 @@ struct gdh_ctx {
 	int *status; /* nonzero: a dependency wait timed out */
 	unsigned epoch;
 	int sweep_grid; /* resident blocks of the persistent sweep */
+	int sweep_grid_lf; /* the same with the relaxation by the element's last
+			    * item (0: not measured) */
 	unsigned *epoch_dev; /* batches: sweep epoch in device memory (graphs) */
 
 	/* batch of instances; graphs of two steps (by state parity) */
@@ -142,7 +144,9 @@ struct gdh_ctx {
 	int sweep_mode; /* 2 clusters, 1 persistent items, 0 one launch per level */
 	int sweep_poll; /* 1: waiting items poll whole traces */
 	int relax_pending; /* the state awaits the relaxation of its last step */
-	int relax_fused; /* the sweep applies the relaxation as it reads its input */
+	int relax_fused; /* 1: the sweep applies the relaxation as it reads its
+			  * input; 2: each element's last item relaxes the
+			  * element's output (LF); 0: a pass after the sweep */
 	/* cluster sweep: host copies of the element vertices (binary32, 12 per
 	 * element) and neighbours (mesh order), taken at creation; per
 	 * cluster (v, elements) a header {first item, n | nx << 8, external

@@ -311,6 +311,7 @@ extern "C" gdh_ctx *gdh_create_ex(const gdh_mesh_desc *mesh,
 	c->nitems = 0;
 	c->relax_pending = 0;
 	c->relax_fused = 0;
+	c->sweep_grid_lf = 0;
 	c->ncl = 0;
 	c->chead = nullptr;
 	c->cdesc = nullptr;

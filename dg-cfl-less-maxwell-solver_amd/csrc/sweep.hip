@@ -117,6 +117,9 @@ __global__ void k_block_inverse(const int2 *__restrict__ items, int nitems,
 			out[10 * i + j] = A[i][10 + j];
 }
 
+__global__ void k_flag_last(struct ItemDesc *__restrict__ desc, const int *__restrict__ last,
+			    int ne);
+
 extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 				const int *items_ve, const int *level_ptr,
 				int nlevels)
@@ -160,6 +163,21 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 		c->nlagged += cls[k] == 3;
 	c->lag_last = c->lag_max = 0;
 	sweep_setup(c);
+	{
+		/* the last item of each element in the level order (descriptor
+		 * pad bit 0): with the relaxation by the element's last item it
+		 * relaxes every velocity of the element */
+		std::vector<int> last(ne, -1);
+		for (int k = 0; k < c->nitems; k++)
+			last[items[k].y] = k;
+		int *dlast = (int *)dmalloc(sizeof(int) * (size_t)ne);
+		HIPCHECK(hipMemcpy(dlast, last.data(), sizeof(int) * (size_t)ne,
+				   hipMemcpyHostToDevice));
+		k_flag_last<<<nblocks(ne, 256), 256, 0, c->stream>>>(c->desc, dlast, ne);
+		HIPCHECK(hipGetLastError());
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		HIPCHECK(hipFree(dlast));
+	}
 	const int mode = c->sweep_mode;
 	c->sweep_mode = mode == 0 ? 0 : 1; /* the capacitance set-up sweeps by items */
 	if (c->nlagged) {
@@ -210,7 +228,9 @@ struct ItemDesc {
 		    * cluster from outside, or 8 + the in-cluster producer;
 		    * bit 28: a consumer lies in another cluster (the item's
 		    * values go to the hand-off words) */
-	int pad; /* cluster order only: the item's level-order index */
+	int pad; /* cluster order: the item's level-order index; level order:
+		  * 1 on the last item of its element (the one that relaxes the
+		  * element when the relaxation is done by the last item) */
 	int nb[4]; /* neighbour element, or the boundary face index */
 	int map[4]; /* 4 bits per face node: the neighbour's local node */
 };
@@ -335,7 +355,7 @@ struct ItemBuf {
 			   * in distinct banks) */
 	double Z[60]; /* after the wait: 2 f - sum_f c_f ML_f u_f */
 	double cf[3];
-	int v, e, nslot, pad;
+	int v, e, nslot, last; /* last: the element's last item (pad bit 0) */
 	int nb[3], map[3], face[3];
 };
 
@@ -561,6 +581,7 @@ __device__ inline double prep_compute(const ItemDesc &d, const PrepLoads<R, NV> 
 		int s = 0;
 		B.v = d.v;
 		B.e = d.e;
+		B.last = d.pad & 1;
 		for (int f = 0; f < 4; f++) {
 			if (!(sf & (1 << f)))
 				continue;
@@ -736,12 +757,13 @@ __device__ inline bool spin_expired(int spins, int *status)
 
 /* sol = D^-1 (z0 - sum_s c_s ML_s X_s) - fv with X[s] the published trace w
  * of slot s's neighbour (fv, z0: the lane's old value and z0); sol goes to
- * fn and w = (1 - theta) fv + theta sol to the hand-off words, each rounded
- * once to R. */
-template <typename R>
-__device__ inline void sweep_finish(ItemBuf &B, const double (&X)[3][36], R fv, double z0,
-				    R *__restrict__ fn, u64 *ll, int ne, unsigned epoch,
-				    const SweepTables &T)
+ * fn (unless the element's last item relaxes and stores it, LF) and
+ * w = (1 - theta) fv + theta sol to the hand-off words, each rounded once to
+ * R. Returns the lane's sol. */
+template <typename R, bool LF = false>
+__device__ inline double sweep_finish(ItemBuf &B, const double (&X)[3][36], R fv, double z0,
+				      R *__restrict__ fn, u64 *ll, int ne, unsigned epoch,
+				      const SweepTables &T)
 {
 	const int t = lane_id();
 	const int iw = t / 10, i = t % 10;
@@ -759,20 +781,111 @@ __device__ inline void sweep_finish(ItemBuf &B, const double (&X)[3][36], R fv, 
 		B.Z[t] = z0 - y;
 	}
 	__syncthreads();
+	double sol = 0.0;
 	if (t < 60) {
 		double sm = 0.0;
 #pragma unroll
 		for (int l = 0; l < 10; l++)
 			sm += B.D[i][l] * B.Z[10 * iw + l];
-		const double sol = sm - (double)fv;
+		sol = sm - (double)fv;
 		const int v = uni(B.v), e = uni(B.e);
 		/* no consumer reads it during the sweep: non-temporal (separate
 		 * relaxation 2 % faster, fused unchanged) */
-		__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
+		if (!LF)
+			__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
 		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
 		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 	}
 	__syncthreads();
+	return sol;
+}
+
+/*
+ * Relaxation by the element's last item (LF). The other items of the
+ * element (its other velocities u) come earlier in the level order, so each
+ * is done once its hand-off words show this sweep's parity; they hold
+ * w_u = (1 - theta) f_u + theta x_u, and with f_u its old value (the
+ * sweep's input, unchanged during the sweep) its transport output is
+ * x_u = (w_u - (1 - theta) f_u) / theta, rounded to R as a stored value
+ * would be (within three units in the last place of x_u, from the rounding
+ * and the parity bit of w_u). With the item's own output x_v every velocity
+ * of the element is relaxed, with the arithmetic of k_relax
+ * (src/dg/dg.c:109-132), and stored. The wait is bounded like the others.
+ */
+template <typename R, int NV>
+__device__ inline void relax_last(int v, int e, double sol, const R *__restrict__ fo,
+				  R *__restrict__ fn, u64 *ll, int ne, unsigned epoch,
+				  const KinPar &kp, int *status)
+{
+	const int t = lane_id();
+	const int tc = min(t, 59), iw = tc / 10, i = tc % 10;
+	typename Handoff<R>::W w[NV];
+	R fu[NV];
+#pragma unroll
+	for (int u = 0; u < NV; u++) {
+		w[u] = ll_load(ll_item<R>(ll, u, e, ne) + tc);
+		fu[u] = fo[((long)u * ne + e) * 60 + tc];
+	}
+	unsigned pend = 0;
+#pragma unroll
+	for (int u = 0; u < NV; u++)
+		if (u != v && !Handoff<R>::fresh(w[u], epoch))
+			pend |= 1u << u;
+	int spins = 0;
+	while (__ballot(pend != 0)) {
+		__builtin_amdgcn_s_sleep(1);
+#pragma unroll
+		for (int u = 0; u < NV; u++)
+			if (pend & (1u << u)) {
+				w[u] = ll_load(ll_item<R>(ll, u, e, ne) + tc);
+				if (Handoff<R>::fresh(w[u], epoch))
+					pend &= ~(1u << u);
+			}
+		if (spin_expired(++spins, status))
+			break;
+	}
+	double x[NV];
+#pragma unroll
+	for (int u = 0; u < NV; u++)
+		x[u] = u == v ? (double)(R)sol
+			      : (double)(R)((Handoff<R>::value(w[u]) - (1.0 - THETA) * (double)fu[u]) /
+					    THETA);
+	double s = 0.0;
+#pragma unroll
+	for (int u = 0; u < NV; u++)
+		s += x[u];
+	double wn[6];
+#pragma unroll
+	for (int q = 0; q < 6; q++)
+		wn[q] = __shfl(s, 10 * q + i, 64);
+	const double ie = kp.ieps ? kp.ieps[e] : 1.0;
+#pragma unroll
+	for (int q = 0; q < 3; q++)
+		wn[q] *= ie;
+	double r[3];
+	maxwell_q_row(wn, iw, r);
+	const double om = kp.omega;
+	if (t < 60) {
+		R *p = fn + (long)e * 60 + t;
+		const long vs = 60L * ne;
+		if (kp.kind == GDH_KIN_D3Q4) {
+			const double l2 = kp.lambda * kp.lambda;
+#pragma unroll
+			for (int u = 0; u < NV; u++) {
+				const double *vv = kp.vi + 3 * u;
+				const double qv = r[0] * vv[0] + r[1] * vv[1] + r[2] * vv[2];
+				__builtin_nontemporal_store(
+					(R)(om * ((s + qv / l2) / kp.nv) + (1. - om) * x[u]), p + u * vs);
+			}
+		} else {
+#pragma unroll
+			for (int u = 0; u < NV; u++) {
+				const double a = s / 6., b = r[u % 3] / (2. * kp.lambda);
+				__builtin_nontemporal_store(
+					(R)(om * (u < 3 ? a + b : a - b) + (1. - om) * x[u]), p + u * vs);
+			}
+		}
+	}
 }
 
 __device__ inline void load_tables(SweepTables &T)
@@ -815,6 +928,14 @@ __global__ __launch_bounds__(64) void k_sweep_level(
 }
 
 /* an ItemDesc held one dword per lane (lane k: dword k), made uniform */
+/* descriptor pad bit 0 on the item last[e] of every element e */
+__global__ void k_flag_last(ItemDesc *__restrict__ desc, const int *__restrict__ last, int ne)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ne && last[e] >= 0)
+		desc[last[e]].pad = 1;
+}
+
 __device__ inline ItemDesc desc_from_lanes(int x)
 {
 	static_assert(sizeof(ItemDesc) == 80, "ItemDesc is 20 dwords");
@@ -870,7 +991,7 @@ __device__ inline unsigned queue_take(unsigned *q)
 #define QUEUE_SHARDS 64
 #define QUEUE_STRIDE 32 /* unsigned words between counters: 128 B */
 #define QUEUE_BYTES (2 * QUEUE_SHARDS * QUEUE_STRIDE * sizeof(unsigned))
-template <typename R, bool FULL_POLL, int NV, bool LB = false, bool QU = false>
+template <typename R, bool FULL_POLL, int NV, bool LB = false, bool QU = false, bool LF = false>
 __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 	const ItemDesc *__restrict__ desc, int nitems, const R *__restrict__ fo,
 	R *__restrict__ fn, const R *__restrict__ xl, const double *__restrict__ dinv,
@@ -969,10 +1090,18 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 					break;
 			}
 			__syncthreads(); /* X complete */
-			sweep_finish<R>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			const double sol = sweep_finish<R, LF>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			/* the element's last item relaxes it now (deferring it to the
+			 * next wait delays that item's finish: 1.02 vs 0.88 ms at 32^3) */
+			if (LF && uni(B[k].last))
+				relax_last<R, NV>(uni(B[k].v), uni(B[k].e), sol, fo, fn, ll, ne, epoch, kp,
+						  status);
 		} else {
 			__syncthreads(); /* X complete */
-			sweep_finish<R>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			const double sol = sweep_finish<R, LF>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			if (LF && uni(B[k].last))
+				relax_last<R, NV>(uni(B[k].v), uni(B[k].e), sol, fo, fn, ll, ne, epoch, kp,
+						  status);
 			if (has_next)
 				nz0 = prep_compute<R, NV, LB>(dn, P, fo, xl, ne, neq, nv, relax_in,
 							      kp, T, B[k ^ 1], W, nfv);
@@ -1428,6 +1557,11 @@ using SweepKernel = void (*)(const ItemDesc *, int, const R *, R *, const R *, c
 template <typename R, bool LB, bool QU>
 static SweepKernel<R> sweep_kernel_q(const gdh_ctx *c)
 {
+	/* the relaxation by the element's last item: whole-trace polling, the
+	 * stored inverses and the static order only (relax_choose) */
+	if (c->relax_fused == 2 && !LB && !QU && c->sweep_poll)
+		return c->kp.nv == 4 ? k_sweep_persistent<R, true, 4, false, false, true>
+				     : k_sweep_persistent<R, true, 6, false, false, true>;
 	if (c->kp.nv == 4)
 		return c->sweep_poll ? k_sweep_persistent<R, true, 4, LB, QU>
 				     : k_sweep_persistent<R, false, 4, LB, QU>;
@@ -1646,7 +1780,7 @@ static void sweep_step_t(gdh_ctx *c, R *fo, R *fn, const R *xl, int relax_in)
 		}
 		return;
 	}
-	const int grid = std::min(c->sweep_grid, nitems);
+	const int grid = std::min(c->relax_fused == 2 ? c->sweep_grid_lf : c->sweep_grid, nitems);
 	const SweepKernel<R> kern = sweep_kernel<R>(c);
 	/* items from the queue (slots alternate between launches) or in the
 	 * static order */
@@ -2494,7 +2628,7 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 	/* unless fused (see relax_choose), and over upwind cycles (lagged
 	 * faces read the neighbour's stored values), the relaxation of the last
 	 * step is applied before the sweep */
-	if (!c->relax_fused)
+	if (c->relax_fused != 1)
 		relax_materialize(c);
 	void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
 	const int rin = c->relax_pending;
@@ -2519,7 +2653,8 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 		c->lag_max = std::max(c->lag_max, c->lag_last);
 	}
 	prof_end(c, 0);
-	c->relax_pending = 1;
+	/* relaxed by the elements' last items, or pending */
+	c->relax_pending = c->relax_fused != 2;
 	c->cur = 1 - c->cur;
 	if (!c->relax_fused)
 		relax_materialize(c);
@@ -2528,39 +2663,57 @@ extern "C" void gdh_relax_step(gdh_ctx *c)
 /*
  * Form of the sweep and of the relaxation, by measurement on this mesh.
  * The item sweep (one item per workgroup, every value handed over through
- * memory) can apply the relaxation as it reads its input: that saves the
- * relaxation kernel's pass over the state but gives every sweep item the
- * loads of all velocities at its nodes and the relaxation arithmetic, which
- * is hidden where the sweep is bound by its dependency chain and not where
- * it is bound by its item rate (measured on MI355X, D3Q4 item sweep per
- * step, fused vs separate: FP64 32^3 0.883 vs 0.911 ms; FP32 32^3 0.875 vs
- * 0.844 ms; 64^3 the cluster sweep wins). The cluster sweep takes a
- * relaxed state. Each candidate (item sweep with the relaxation separate or
- * fused, cluster sweep with it separate) is timed twice, five steps after
- * one each time, from the current state into the other buffer, and its
- * better time counts; nothing of the state changes. GDN_SWEEP and
- * GDN_RELAX_FUSED=0|1 restrict the candidates.
+ * memory) can apply the relaxation as it reads its input (fused): that
+ * saves the relaxation kernel's pass over the state but gives every sweep
+ * item the loads of all velocities at its nodes and the relaxation
+ * arithmetic. Or each element's last item in the level order relaxes the
+ * whole element once its other velocities are done (LF, binary64, only on
+ * request, GDN_RELAX_FUSED=2): the other items store no state, the last one
+ * reads their published traces and old values and stores every velocity
+ * relaxed, and there is no pass; its extra work sits on the item-turnover
+ * path and it was slower than the best other form at every size measured
+ * (item sweep per step: 8^3 0.120 vs 0.108 ms fused, 16^3 0.247 vs 0.227
+ * fused, 24^3 0.503 vs 0.479 fused and 0.511 separate, 32^3 0.883 vs 0.849
+ * separate, 64^3 6.58 vs 5.95 separate). The cluster sweep takes a relaxed
+ * state. Each candidate (item sweep with the relaxation separate or fused,
+ * cluster sweep with it separate) is timed twice, five steps after one each
+ * time, from the current state into the other buffer, and its better time
+ * counts; nothing of the state changes. GDN_SWEEP and GDN_RELAX_FUSED=0|1|2
+ * restrict the candidates.
  */
 static void relax_choose(gdh_ctx *c)
 {
 	const char *env = getenv("GDN_RELAX_FUSED");
 	/* lagged faces and the projector boundary state read stored element
-	 * values, which must hold the relaxation: no fused form */
-	const int fz_only = (c->nlagged || c->kp.fr_kind == GDH_FR_PROJECTOR ||
-			     (env && !strcmp(env, "0"))) ? 0
-			    : (env && !strcmp(env, "1"))		    ? 1
-									    : -1;
-	int cand[3][2], nc = 0; /* (sweep mode, fused) */
+	 * values, which must hold the relaxation (lagged faces: the transport
+	 * output): no fused or LF form; LF also needs binary64 (its recovery of
+	 * the other velocities' output from the published traces rounds at the
+	 * state precision), the stored inverses, whole-trace polling and the
+	 * static item order */
+	const int fixed = env ? atoi(env) : -1;
+	const bool any_fused = !(c->nlagged || c->kp.fr_kind == GDH_FR_PROJECTOR);
+	const bool lf_ok = any_fused && !c->fp32 && !c->kp.lb_p && c->sweep_poll && !c->sweep_queue;
+	if (fixed == 2 && !lf_ok) {
+		fprintf(stderr, "gdon3d-hip: GDN_RELAX_FUSED=2 needs binary64, the stored inverses, "
+				"whole-trace polling, the static order and no upwind cycles or "
+				"projector boundary\n");
+		abort();
+	}
+	int cand[4][2], nc = 0; /* (sweep mode, relaxation form) */
 	for (int mode = 0; mode <= 2; mode++) {
 		if (c->sweep_mode >= 0 ? mode != c->sweep_mode : mode == 0)
 			continue;
 		if (mode == 2 && !c->ncl)
 			continue;
-		for (int fused = 0; fused < 2; fused++) {
-			if ((fz_only >= 0 && fused != fz_only) || (mode == 2 && fused))
+		for (int form = 0; form < 3; form++) {
+			if (fixed >= 0 && form != fixed)
+				continue;
+			if (form && (mode != 1 || !any_fused))
+				continue;
+			if (form == 2 && (!lf_ok || fixed != 2))
 				continue;
 			cand[nc][0] = mode;
-			cand[nc][1] = fused;
+			cand[nc][1] = form;
 			nc++;
 		}
 	}
@@ -2568,6 +2721,13 @@ static void relax_choose(gdh_ctx *c)
 		fprintf(stderr, "gdon3d-hip: no form of the sweep fits GDN_SWEEP / GDN_RELAX_FUSED\n");
 		abort();
 	}
+	/* the LF kernel's own residency (its registers differ) */
+	for (int k = 0; k < nc; k++)
+		if (cand[k][1] == 2 && !c->sweep_grid_lf) {
+			c->relax_fused = 2;
+			c->sweep_grid_lf = grid_cap(std::min(c->sweep_grid, sweep_residency<double>(c)));
+			c->relax_fused = 0;
+		}
 	int best = 0;
 	if (nc > 1) {
 		void *fo = c->w[c->cur], *fn = c->w[1 - c->cur];
@@ -2576,14 +2736,15 @@ static void relax_choose(gdh_ctx *c)
 		/* two rounds over the candidates, the best time of each kept: the
 		 * first launches after set-up run slower, which would otherwise
 		 * count against whichever candidate is timed first */
-		float tms[3] = { 1e30f, 1e30f, 1e30f };
+		float tms[4] = { 1e30f, 1e30f, 1e30f, 1e30f };
 		for (int round = 0; round < 2; round++)
 			for (int k = 0; k < nc; k++) {
 				c->sweep_mode = cand[k][0];
+				c->relax_fused = cand[k][1];
 				for (int r = 0; r < 6; r++) {
 					if (r == 1)
 						HIPCHECK(hipEventRecord(c->ev0, c->stream));
-					sweep_step(c, fo, fn, nullptr, cand[k][1]);
+					sweep_step(c, fo, fn, nullptr, cand[k][1] == 1);
 					if (!cand[k][1])
 						relax_launch(c, fn, 10L * c->ne);
 				}
@@ -2592,7 +2753,7 @@ static void relax_choose(gdh_ctx *c)
 				float ms = 0;
 				HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
 				if (getenv("GDN_VERBOSE"))
-					fprintf(stderr, "gdon3d-hip: sweep form %d, relaxation fused %d: "
+					fprintf(stderr, "gdon3d-hip: sweep form %d, relaxation form %d: "
 							"%.3f ms per step\n",
 						cand[k][0], cand[k][1], ms / 5);
 				tms[k] = std::min(tms[k], ms);

@@ -3,9 +3,10 @@ the C ABI: upwind block sweep + node relaxation) against the oracle's
 assembled theta-scheme with sparse LU (src/dg/relaxation.c:24-185).
 
 Every oracle comparison names the form of the transport sweep it checks
-(GDN_SWEEP = level | item | cluster, GDN_RELAX_FUSED = 0 | 1) and asserts
-that the library ran that form; the benchmarked form is the item sweep with
-the relaxation fused into it (item, 1).
+(GDN_SWEEP = level | item | cluster, GDN_RELAX_FUSED = 0 | 1 | 2: the
+relaxation as a pass of its own, fused into the next sweep's reads, or done
+by each element's last item) and asserts that the library ran that form;
+the benchmark runs whichever of them the set-up timing picks.
 
 Tolerance: relative L2 <= 1e-12 on the full micro state wn (FP64)."""
 import os
@@ -22,7 +23,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # (GDN_SWEEP, GDN_RELAX_FUSED): every form the library can run; the cluster
 # and level sweeps take a relaxed state (relaxation as a pass of its own)
-FORMS = [("item", "1"), ("item", "0"), ("cluster", "0"), ("level", "0")]
+FORMS = [("item", "1"), ("item", "0"), ("item", "2"), ("cluster", "0"), ("level", "0")]
 FORM_IDS = [f"{s}-fused{f}" for s, f in FORMS]
 
 _ORACLE_CACHE = {}
@@ -438,3 +439,48 @@ def test_relaxation_pass_pair_equals_wave(kin, precision, monkeypatch):
           f"{np.abs(out['pair'] - out['wave']).max():.3e}", flush=True)
     assert np.all(np.isfinite(out["pair"]))
     assert np.array_equal(out["pair"], out["wave"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kin,eps", [("d3q4", False), ("d3q4", True), ("d3q6", False)])
+def test_last_item_relaxation_equals_separate(kin, eps, monkeypatch):
+    """The relaxation done inside the sweep by each element's last item
+    (GDN_RELAX_FUSED=2: it takes its sibling velocities' transport output
+    from their published traces and old values, x = (w - (1 - theta) f) /
+    theta, relaxes the element and stores it; no pass) against the pass after
+    the sweep (GDN_RELAX_FUSED=0): the recovered output differs from the
+    stored one by at most a few units in the last place, so whole solves
+    agree to relative L2 <= 1e-13 (binary64) through several steps, the L2
+    error, the download and a second advance; jittered mesh, with and
+    without permittivity per element."""
+    path = kuhn_msh(6, 0.1, 43)
+    G.set_cavity_mode(1, 1, 1, (1, -1, 0))
+    monkeypatch.setenv("GDN_SWEEP", "item")
+    ne = 6 * 6 ** 3
+    e = np.random.default_rng(9).uniform(1.0, 4.0, ne) if eps else None
+    out = {}
+    for form in ("0", "2"):
+        monkeypatch.setenv("GDN_RELAX_FUSED", form)
+        mesh = G.Mesh(path=path)
+        sim = G.Simulation(mesh, G.Model("cavity_mode", kinetic=kin))
+        sim.set_time(0.4, 0.0, 5.0)
+        st = G.Stepper(sim, eps=e)
+        assert G.relax_fused(st) == int(form)
+        assert G.sweep_info(st)["form"] == "item"
+        half = sim.s.itermax // 2
+        st.advance(half)
+        err_mid = st.error_l2()
+        st.finish()
+        mid = sim.wn()
+        st.advance(sim.s.itermax - half)
+        st.sync()
+        err = st.error_l2()
+        st.finish()
+        out[form] = (mid, sim.wn(), err_mid, err)
+        st.free()
+        sim.free()
+    a, b = out["0"], out["2"]
+    print(f"last-item vs pass relaxation {kin} eps={eps}: mid {rel_l2(b[0], a[0]):.3e} "
+          f"end {rel_l2(b[1], a[1]):.3e}", flush=True)
+    assert rel_l2(b[0], a[0]) <= 1e-13 and rel_l2(b[1], a[1]) <= 1e-13
+    assert abs(b[2] - a[2]) <= 1e-13 * a[2] and abs(b[3] - a[3]) <= 1e-13 * a[3]
