@@ -466,7 +466,9 @@ __device__ inline void prep_issue(int it, const ItemDesc &d, const R *__restrict
 		for (int u = 0; u < NV; u++)
 			P.fr[u] = t < 60 ? fo[((long)u * ne + d.e) * 60 + t] : (R)0;
 	} else {
-		P.fr[0] = t < 60 ? fo[((long)d.v * ne + d.e) * 60 + t] : (R)0;
+		/* the item's own values, read by this item only: non-temporal */
+		P.fr[0] = t < 60 ? __builtin_nontemporal_load(fo + ((long)d.v * ne + d.e) * 60 + t)
+				 : (R)0;
 #pragma unroll
 		for (int u = 1; u < NV; u++)
 			P.fr[u] = (R)0;
@@ -789,12 +791,13 @@ __device__ inline double sweep_finish(ItemBuf &B, const double (&X)[3][36], R fv
 			sm += B.D[i][l] * B.Z[10 * iw + l];
 		sol = sm - (double)fv;
 		const int v = uni(B.v), e = uni(B.e);
+		/* the hand-off word first: the downstream items wait for it */
+		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
+		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 		/* no consumer reads it during the sweep: non-temporal (separate
 		 * relaxation 2 % faster, fused unchanged) */
 		if (!LF)
 			__builtin_nontemporal_store((R)sol, fn + ((long)v * ne + e) * 60 + t);
-		const R wr = (R)((1.0 - THETA) * (double)fv + THETA * sol);
-		Handoff<R>::put(ll_item<R>(ll, v, e, ne), t, wr, epoch);
 	}
 	__syncthreads();
 	return sol;
@@ -1089,16 +1092,23 @@ __global__ __launch_bounds__(64, LB ? 4 : 5) void k_sweep_persistent(
 				if (spin_expired(++spins, status))
 					break;
 			}
+			/* the finish at the highest wave priority: downstream items
+			 * wait for its hand-off words (32^3 binary64 fused 3 %
+			 * faster) */
+			__builtin_amdgcn_s_setprio(3);
 			__syncthreads(); /* X complete */
 			const double sol = sweep_finish<R, LF>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			__builtin_amdgcn_s_setprio(0);
 			/* the element's last item relaxes it now (deferring it to the
 			 * next wait delays that item's finish: 1.02 vs 0.88 ms at 32^3) */
 			if (LF && uni(B[k].last))
 				relax_last<R, NV>(uni(B[k].v), uni(B[k].e), sol, fo, fn, ll, ne, epoch, kp,
 						  status);
 		} else {
+			__builtin_amdgcn_s_setprio(3);
 			__syncthreads(); /* X complete */
 			const double sol = sweep_finish<R, LF>(B[k], X, fv, z0, fn, ll, ne, epoch, T);
+			__builtin_amdgcn_s_setprio(0);
 			if (LF && uni(B[k].last))
 				relax_last<R, NV>(uni(B[k].v), uni(B[k].e), sol, fo, fn, ll, ne, epoch, kp,
 						  status);
@@ -1441,8 +1451,13 @@ __global__ __launch_bounds__(64, 4) void k_sweep_cluster(
 				if (spin_expired(++spins, status))
 					break;
 			}
+			/* the item at the highest wave priority, the waits at the
+			 * lowest: blocks with work go first (64^3 jittered sweep
+			 * 4.27 -> 4.05 ms binary64, 4.09 -> 3.90 ms binary32) */
+			__builtin_amdgcn_s_setprio(3);
 			cl_item<R>(d, PA[j], fo, xl, fn, ll, ne, neq, nv, epoch, T, W, Wc, Xc,
 				   &Wc[j][0]);
+			__builtin_amdgcn_s_setprio(0);
 		}
 #pragma unroll
 		for (int j = 0; j < CL_MAX; j++)
