@@ -29,6 +29,7 @@
 #include "ctx.h"
 
 #include <numeric>
+#include <thread>
 
 static uint64_t spread3(uint64_t x) /* 21 bits -> every third bit */
 {
@@ -177,6 +178,153 @@ static void group_graph(int ng, int ne, const std::vector<int> &g, Dep dep,
 		}
 }
 
+/* a stable counting sort of the elements in `in` by key[e] (keys below
+ * nkey): `out` lists them by key, in their order in `in` for equal keys */
+static void sort_by_key(const std::vector<int> &in, const std::vector<int> &key, int nkey,
+			std::vector<int> &cnt, std::vector<int> &out)
+{
+	cnt.assign((size_t)nkey + 1, 0);
+	for (const int e : in)
+		cnt[key[e] + 1]++;
+	for (int k = 0; k < nkey; k++)
+		cnt[k + 1] += cnt[k];
+	out.resize(in.size());
+	for (const int e : in)
+		out[cnt[key[e]]++] = e;
+}
+
+struct CItem {
+	int lev, v, key, first, n;
+};
+
+/* The clusters of one velocity: pieces of the base clusters (single
+ * elements where a base cluster lies on a cycle of the cluster graph),
+ * their members in upwind-level order (mem, from cis[].first) and their
+ * levels in the cluster graph. lev: upwind level of every item (v, e).
+ * False when the cluster graph is cyclic. */
+static bool velocity_clusters(int v, int ne, int nv, int nbase, const std::vector<int> &bc,
+			      const int *nbr, const uint8_t *cls, const std::vector<int> &lev,
+			      int nlevels, std::vector<CItem> &cis, std::vector<int> &mem,
+			      int &levels)
+{
+	/* the producer across face f of element e, or -1 */
+	auto dep = [&](int e, int f) -> int {
+		const int nb = nbr[4L * e + f];
+		return nb >= 0 && cls[(4L * e + f) * nv + v] == 2 ? nb : -1;
+	};
+	std::vector<int> ap, aj, comp, grp(ne), cid(ne), order, lorder(ne), ckey, cnt;
+	std::vector<int> clev, indeg, queue;
+	/* the elements in upwind-level order (level, then element index) */
+	const std::vector<int> lv(lev.begin() + (size_t)v * ne, lev.begin() + (size_t)(v + 1) * ne);
+	{
+		std::vector<int> all(ne);
+		std::iota(all.begin(), all.end(), 0);
+		sort_by_key(all, lv, nlevels, cnt, lorder);
+	}
+	group_graph(nbase, ne, bc, dep, ap, aj);
+	const std::vector<int> csize = scc(nbase, ap, aj, comp);
+	/* groups: base clusters, or single elements inside a cycle */
+	for (int e = 0; e < ne; e++)
+		grp[e] = csize[comp[bc[e]]] > 1 ? nbase + e : bc[e];
+	/* by group, each group in upwind-level order */
+	sort_by_key(lorder, grp, nbase + ne, cnt, order);
+	/* cut each group in level order: at most CL_MAX elements and
+	 * CL_XMAX inflow faces from outside the piece */
+	int ncid = 0, nin = 0, nxc = 0; /* current piece: elements, faces from outside */
+	for (int k = 0; k < ne; k++) {
+		const int e = order[k];
+		const bool same = k > 0 && grp[e] == grp[order[k - 1]];
+		/* inflow faces of e from outside the current piece, were e
+		 * added to it (a producer in the same group comes earlier in
+		 * `order`, so its piece is already set) */
+		int ex = 0, all = 0;
+		for (int f = 0; f < 4; f++) {
+			const int u = dep(e, f);
+			if (u < 0)
+				continue;
+			all++;
+			ex += !(same && grp[u] == grp[e] && cid[u] == ncid - 1);
+		}
+		if (!same || nin == CL_MAX || nxc + ex > CL_XMAX) {
+			ncid++;
+			ckey.push_back(bc[e]);
+			nin = 0;
+			nxc = 0;
+			ex = all;
+		}
+		cid[e] = ncid - 1;
+		nin++;
+		nxc += ex;
+	}
+	/* flow-following merge, in upwind-level order: a single-element
+	 * piece whose producers all lie in one piece of fewer than CL_MAX
+	 * elements joins it. Its in-edges then all come from inside that
+	 * piece, so a cycle through the merged piece would be one through
+	 * the piece and the single element before: none can form. This
+	 * recovers clusters where grid cells cut across the flow
+	 * (unstructured meshes). */
+	std::vector<int> psize(ncid, 0);
+	for (int e = 0; e < ne; e++)
+		psize[cid[e]]++;
+	for (const int e : lorder) {
+		if (psize[cid[e]] != 1)
+			continue;
+		int c = -1, np = 0;
+		bool one = true;
+		for (int f = 0; f < 4; f++) {
+			const int u = dep(e, f);
+			if (u < 0)
+				continue;
+			np++;
+			if (c < 0)
+				c = cid[u];
+			else if (cid[u] != c)
+				one = false;
+		}
+		if (!one || np == 0 || psize[c] >= CL_MAX)
+			continue;
+		psize[cid[e]]--;
+		cid[e] = c;
+		psize[c]++;
+	}
+	/* members of each piece in upwind-level order */
+	sort_by_key(lorder, cid, ncid, cnt, order);
+	/* longest path over the pieces (Kahn); all placed = acyclic */
+	group_graph(ncid, ne, cid, dep, ap, aj);
+	clev.assign(ncid, 0);
+	indeg.assign(ncid, 0);
+	for (int x : aj)
+		indeg[x]++;
+	for (int g = 0; g < ncid; g++)
+		if (!indeg[g])
+			queue.push_back(g);
+	for (size_t h = 0; h < queue.size(); h++) {
+		const int g = queue[h];
+		for (int k = ap[g]; k < ap[g + 1]; k++) {
+			const int x = aj[k];
+			clev[x] = std::max(clev[x], clev[g] + 1);
+			if (--indeg[x] == 0)
+				queue.push_back(x);
+		}
+	}
+	if ((int)queue.size() != ncid)
+		return false;
+	/* pieces are runs of `order` */
+	levels = 0;
+	for (int k = 0; k < ne;) {
+		int k1 = k;
+		while (k1 < ne && cid[order[k1]] == cid[order[k]])
+			k1++;
+		const int g = cid[order[k]];
+		cis.push_back({ clev[g], v, ckey[g], (int)mem.size(), k1 - k });
+		for (int j = k; j < k1; j++)
+			mem.push_back(order[j]);
+		levels = std::max(levels, clev[g] + 1);
+		k = k1;
+	}
+	return true;
+}
+
 bool cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const std::vector<int> &rng,
 		  const uint8_t *cls, const int *items_ve, const int *level_ptr, int nlevels,
 		  ClusterPlan &P)
@@ -190,196 +338,132 @@ bool cluster_plan(int ne, int nv, const float *vtx, const int *nbr, const std::v
 			lev[items_ve[k]] = L;
 			ipos[items_ve[k]] = k;
 		}
-	struct CItem {
-		int lev, v, key, first, n;
-	};
+	/* the velocities' clusters, one host thread each (the work of one
+	 * velocity is independent of the others'), joined in velocity order */
+	std::vector<std::vector<CItem>> cv(nv);
+	std::vector<std::vector<int>> mv(nv);
+	std::vector<int> lv(nv, 0);
+	std::vector<char> ok(nv, 0);
+	{
+		std::vector<std::thread> th;
+		for (int v = 0; v < nv; v++)
+			th.emplace_back([&, v] {
+				ok[v] = velocity_clusters(v, ne, nv, nbase, bc, nbr, cls, lev, nlevels,
+							  cv[v], mv[v], lv[v]);
+			});
+		for (auto &t : th)
+			t.join();
+	}
 	std::vector<CItem> cis;
-	std::vector<int> mem; /* member elements of the cluster items */
-	std::vector<int> ap, aj, comp, grp(ne), cid(ne), order(ne), ckey;
-	std::vector<int> clev, indeg, queue;
+	std::vector<int> mem;
 	P.levels = 0;
 	P.multi = 0;
 	for (int v = 0; v < nv; v++) {
-		/* the producer across face f of element e, or -1 */
-		auto dep = [&](int e, int f) -> int {
-			const int nb = nbr[4L * e + f];
-			return nb >= 0 && cls[(4L * e + f) * nv + v] == 2 ? nb : -1;
-		};
-		group_graph(nbase, ne, bc, dep, ap, aj);
-		const std::vector<int> csize = scc(nbase, ap, aj, comp);
-		/* groups: base clusters, or single elements inside a cycle */
-		for (int e = 0; e < ne; e++)
-			grp[e] = csize[comp[bc[e]]] > 1 ? nbase + e : bc[e];
-		std::iota(order.begin(), order.end(), 0);
-		std::sort(order.begin(), order.end(), [&](int a, int b) {
-			if (grp[a] != grp[b])
-				return grp[a] < grp[b];
-			const int la = lev[(size_t)v * ne + a], lb = lev[(size_t)v * ne + b];
-			return la != lb ? la < lb : a < b;
-		});
-		/* cut each group in level order: at most CL_MAX elements and
-		 * CL_XMAX inflow faces from outside the piece */
-		int ncid = 0, cnt = 0, nxc = 0; /* current piece: elements, faces from outside */
-		ckey.clear();
-		for (int k = 0; k < ne; k++) {
-			const int e = order[k];
-			const bool same = k > 0 && grp[e] == grp[order[k - 1]];
-			/* inflow faces of e from outside the current piece, were e
-			 * added to it (a producer in the same group comes earlier in
-			 * `order`, so its piece is already set) */
-			int ex = 0, all = 0;
-			for (int f = 0; f < 4; f++) {
-				const int u = dep(e, f);
-				if (u < 0)
-					continue;
-				all++;
-				ex += !(same && grp[u] == grp[e] && cid[u] == ncid - 1);
-			}
-			if (!same || cnt == CL_MAX || nxc + ex > CL_XMAX) {
-				ncid++;
-				ckey.push_back(bc[e]);
-				cnt = 0;
-				nxc = 0;
-				ex = all;
-			}
-			cid[e] = ncid - 1;
-			cnt++;
-			nxc += ex;
-		}
-		/* flow-following merge, in upwind-level order: a single-element
-		 * piece whose producers all lie in one piece of fewer than CL_MAX
-		 * elements joins it. Its in-edges then all come from inside that
-		 * piece, so a cycle through the merged piece would be one through
-		 * the piece and the single element before: none can form. This
-		 * recovers clusters where grid cells cut across the flow
-		 * (unstructured meshes). */
-		{
-			std::vector<int> psize(ncid, 0), lorder(ne);
-			for (int e = 0; e < ne; e++)
-				psize[cid[e]]++;
-			std::iota(lorder.begin(), lorder.end(), 0);
-			std::sort(lorder.begin(), lorder.end(), [&](int a, int b) {
-				const int la = lev[(size_t)v * ne + a], lb = lev[(size_t)v * ne + b];
-				return la != lb ? la < lb : a < b;
-			});
-			for (const int e : lorder) {
-				if (psize[cid[e]] != 1)
-					continue;
-				int c = -1, np = 0;
-				bool one = true;
-				for (int f = 0; f < 4; f++) {
-					const int u = dep(e, f);
-					if (u < 0)
-						continue;
-					np++;
-					if (c < 0)
-						c = cid[u];
-					else if (cid[u] != c)
-						one = false;
-				}
-				if (!one || np == 0 || psize[c] >= CL_MAX)
-					continue;
-				psize[cid[e]]--;
-				cid[e] = c;
-				psize[c]++;
-			}
-			/* members of each piece in upwind-level order */
-			std::sort(order.begin(), order.end(), [&](int a, int b) {
-				if (cid[a] != cid[b])
-					return cid[a] < cid[b];
-				const int la = lev[(size_t)v * ne + a], lb = lev[(size_t)v * ne + b];
-				return la != lb ? la < lb : a < b;
-			});
-		}
-		/* longest path over the pieces (Kahn); all placed = acyclic */
-		group_graph(ncid, ne, cid, dep, ap, aj);
-		clev.assign(ncid, 0);
-		indeg.assign(ncid, 0);
-		for (int x : aj)
-			indeg[x]++;
-		queue.clear();
-		for (int g = 0; g < ncid; g++)
-			if (!indeg[g])
-				queue.push_back(g);
-		for (size_t h = 0; h < queue.size(); h++) {
-			const int g = queue[h];
-			for (int k = ap[g]; k < ap[g + 1]; k++) {
-				const int x = aj[k];
-				clev[x] = std::max(clev[x], clev[g] + 1);
-				if (--indeg[x] == 0)
-					queue.push_back(x);
-			}
-		}
-		if ((int)queue.size() != ncid)
+		if (!ok[v])
 			return false;
-		/* pieces are runs of `order` */
-		for (int k = 0; k < ne;) {
-			int k1 = k;
-			while (k1 < ne && cid[order[k1]] == cid[order[k]])
-				k1++;
-			const int g = cid[order[k]];
-			cis.push_back({ clev[g], v, ckey[g], (int)mem.size(), k1 - k });
-			for (int j = k; j < k1; j++)
-				mem.push_back(order[j]);
-			P.levels = std::max(P.levels, clev[g] + 1);
-			k = k1;
+		const int off = (int)mem.size();
+		for (CItem q : cv[v]) {
+			q.first += off;
+			cis.push_back(q);
 		}
+		mem.insert(mem.end(), mv[v].begin(), mv[v].end());
+		P.levels = std::max(P.levels, lv[v]);
 	}
-	std::sort(cis.begin(), cis.end(), [](const CItem &a, const CItem &b) {
-		if (a.lev != b.lev)
-			return a.lev < b.lev;
-		if (a.v != b.v)
-			return a.v < b.v;
-		return a.key != b.key ? a.key < b.key : a.first < b.first;
-	});
-	P.head.resize(cis.size());
+	/* order (level, velocity, base cluster, first member): cis is in
+	 * first-member order, so stable counting sorts by the other three keys
+	 * from the last to the first give it */
+	{
+		std::vector<int> idx(cis.size()), k1(cis.size()), cnt, tmp;
+		std::iota(idx.begin(), idx.end(), 0);
+		for (size_t i = 0; i < cis.size(); i++)
+			k1[i] = cis[i].key;
+		sort_by_key(idx, k1, nbase, cnt, tmp);
+		for (size_t i = 0; i < cis.size(); i++)
+			k1[i] = cis[i].v;
+		sort_by_key(tmp, k1, nv, cnt, idx);
+		for (size_t i = 0; i < cis.size(); i++)
+			k1[i] = cis[i].lev;
+		sort_by_key(idx, k1, P.levels, cnt, tmp);
+		std::vector<CItem> sorted(cis.size());
+		for (size_t i = 0; i < cis.size(); i++)
+			sorted[i] = cis[tmp[i]];
+		cis.swap(sorted);
+	}
+	const size_t ncl = cis.size();
+	P.head.resize(ncl);
 	P.perm.resize((size_t)nv * ne);
 	P.code.assign((size_t)nv * ne, 0);
-	std::vector<int> jloc(ne, -1);
-	long pos = 0;
-	for (size_t ci = 0; ci < cis.size(); ci++) {
-		const CItem &q = cis[ci];
-		for (int j = 0; j < q.n; j++)
-			jloc[mem[q.first + j]] = j;
-		int nx = 0;
-		unsigned xb[2] = { 0, 0 };
-		for (int j = 0; j < q.n; j++) {
-			const int e = mem[q.first + j];
-			int code = 0;
-			for (int f = 0; f < 4; f++) {
-				const int nb = nbr[4L * e + f];
-				if (nb < 0 || cls[(4L * e + f) * nv + q.v] != 2)
-					continue;
-				int nib;
-				if (jloc[nb] >= 0 && jloc[nb] < j &&
-				    mem[q.first + jloc[nb]] == nb) {
-					nib = 8 + jloc[nb];
-				} else {
-					if (nx == CL_XMAX)
-						return false;
-					nib = nx;
-					xb[nx / 4] |= (unsigned)(4 * j + f) << (8 * (nx % 4));
-					nx++;
+	std::vector<long> start(ncl + 1, 0);
+	for (size_t ci = 0; ci < ncl; ci++)
+		start[ci + 1] = start[ci] + cis[ci].n;
+	/* each cluster's face codes and header, on the host's threads (ranges
+	 * of clusters; each thread marks its cluster's members in its own
+	 * element array) */
+	const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	std::vector<char> fine(nth, 1);
+	std::vector<long> multi(nth, 0);
+	auto codes = [&](int w) {
+		std::vector<int> jloc(ne, -1);
+		const size_t c0 = ncl * w / nth, c1 = ncl * (w + 1) / nth;
+		for (size_t ci = c0; ci < c1; ci++) {
+			const CItem &q = cis[ci];
+			const long pos = start[ci];
+			for (int j = 0; j < q.n; j++)
+				jloc[mem[q.first + j]] = j;
+			int nx = 0;
+			unsigned xb[2] = { 0, 0 };
+			for (int j = 0; j < q.n; j++) {
+				const int e = mem[q.first + j];
+				int code = 0;
+				for (int f = 0; f < 4; f++) {
+					const int nb = nbr[4L * e + f];
+					if (nb < 0 || cls[(4L * e + f) * nv + q.v] != 2)
+						continue;
+					int nib;
+					if (jloc[nb] >= 0 && jloc[nb] < j &&
+					    mem[q.first + jloc[nb]] == nb) {
+						nib = 8 + jloc[nb];
+					} else {
+						if (nx == CL_XMAX) {
+							fine[w] = 0;
+							return;
+						}
+						nib = nx;
+						xb[nx / 4] |= (unsigned)(4 * j + f) << (8 * (nx % 4));
+						nx++;
+					}
+					code |= nib << (4 * f);
 				}
-				code |= nib << (4 * f);
+				/* bit 16: some downstream neighbour lies outside the
+				 * cluster, so the item's values go to the hand-off
+				 * words */
+				for (int f = 0; f < 4; f++) {
+					const int nb = nbr[4L * e + f];
+					if (nb >= 0 && cls[(4L * e + f) * nv + q.v] == 1 && jloc[nb] < 0)
+						code |= 1 << 16;
+				}
+				P.perm[pos + j] = ipos[(size_t)q.v * ne + e];
+				P.code[pos + j] = code;
 			}
-			/* bit 16: some downstream neighbour lies outside the cluster,
-			 * so the item's values go to the hand-off words */
-			for (int f = 0; f < 4; f++) {
-				const int nb = nbr[4L * e + f];
-				if (nb >= 0 && cls[(4L * e + f) * nv + q.v] == 1 && jloc[nb] < 0)
-					code |= 1 << 16;
-			}
-			P.perm[pos + j] = ipos[(size_t)q.v * ne + e];
-			P.code[pos + j] = code;
+			for (int j = 0; j < q.n; j++)
+				jloc[mem[q.first + j]] = -1;
+			P.head[ci] = make_int4((int)pos, q.n | nx << 8, (int)xb[0], (int)xb[1]);
+			multi[w] += q.n > 1 ? q.n : 0;
 		}
-		for (int j = 0; j < q.n; j++)
-			jloc[mem[q.first + j]] = -1;
-		P.head[ci] = make_int4((int)pos, q.n | nx << 8, (int)xb[0], (int)xb[1]);
-		P.multi += q.n > 1 ? q.n : 0;
-		pos += q.n;
+	};
+	{
+		std::vector<std::thread> th;
+		for (int w = 0; w < nth; w++)
+			th.emplace_back(codes, w);
+		for (auto &t : th)
+			t.join();
 	}
-	return pos == (long)nv * ne;
+	for (int w = 0; w < nth; w++) {
+		if (!fine[w])
+			return false;
+		P.multi += multi[w];
+	}
+	return start[ncl] == (long)nv * ne;
 }
 
 bool cluster_plan(const gdh_ctx *c, const uint8_t *cls, const int *items_ve,

@@ -216,6 +216,9 @@ uint8_t *gdn_upwind_classes(const gdn_mesh *msh, const gdn_model *mdl)
 	uint8_t *cls = calloc((size_t)ne * NFA * nv, 1);
 	if (!cls)
 		gdn_fatal("out of memory");
+	/* faces on the host's threads: each face writes the classes of its
+	 * own two (element, face) slots */
+#pragma omp parallel for schedule(static)
 	for (int F = 0; F < msh->nbfaces; F++) {
 		const int ieL = msh->face2elem[E2F * F + 0];
 		const int ifL = msh->face2elem[E2F * F + 1];
@@ -339,15 +342,20 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 		       int *items, int *level_ptr, int *nlevels)
 {
 	const size_t n1 = nelem ? nelem : 1;
-	int *indeg = malloc(sizeof(int) * n1);
-	int *queue = malloc(sizeof(int) * n1);
-	int *color = malloc(sizeof(int) * n1);
-	int *stack = malloc(sizeof(int) * 2 * n1);
 	int *level = malloc(sizeof(int) * (size_t)nv * n1);
-	if (!indeg || !queue || !color || !stack || !level)
+	if (!level)
 		gdn_fatal("out of memory");
 	int maxlev = -1, nlag = 0;
+	/* the velocities on the host's threads: velocity v reads and lags only
+	 * the classes of v and writes only its own levels */
+#pragma omp parallel for schedule(static, 1) reduction(max : maxlev) reduction(+ : nlag)
 	for (int v = 0; v < nv; v++) {
+		int *indeg = malloc(sizeof(int) * n1);
+		int *queue = malloc(sizeof(int) * n1);
+		int *color = malloc(sizeof(int) * n1);
+		int *stack = malloc(sizeof(int) * 2 * n1);
+		if (!indeg || !queue || !color || !stack)
+			gdn_fatal("out of memory");
 		int *lev = level + (size_t)v * nelem;
 		if (kahn_levels(nelem, nv, v, nbr, cls, indeg, queue, lev) != nelem) {
 			nlag += lag_back_edges(nelem, nv, v, nbr, cls, indeg, color, stack);
@@ -357,9 +365,13 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 		for (int e = 0; e < nelem; e++)
 			if (lev[e] > maxlev)
 				maxlev = lev[e];
+		free(indeg);
+		free(queue);
+		free(color);
+		free(stack);
 	}
 	/* counting sort by level; (v, e) increasing inside a level */
-	const int nl = maxlev + 1;
+	const int nl = maxlev >= 0 ? maxlev + 1 : 0;
 	for (int L = 0; L <= nl; L++)
 		level_ptr[L] = 0;
 	for (int v = 0; v < nv; v++)
@@ -367,7 +379,7 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 			level_ptr[level[(size_t)v * nelem + e] + 1]++;
 	for (int L = 0; L < nl; L++)
 		level_ptr[L + 1] += level_ptr[L];
-	int *pos = malloc(sizeof(int) * (size_t)(nl ? nl : 1));
+	int *pos = malloc(sizeof(int) * (size_t)(nl > 0 ? nl : 1));
 	if (!pos)
 		gdn_fatal("out of memory");
 	for (int L = 0; L < nl; L++)
@@ -377,10 +389,6 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 			items[pos[level[(size_t)v * nelem + e]]++] = v * nelem + e;
 	free(pos);
 	*nlevels = nl;
-	free(indeg);
-	free(queue);
-	free(color);
-	free(stack);
 	free(level);
 	return nlag;
 }
