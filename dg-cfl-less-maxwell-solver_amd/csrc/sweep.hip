@@ -6,6 +6,8 @@
 #include "ctx.h"
 #include "learned_block.h"
 
+#include <chrono>
+
 using namespace gdh;
 
 /* ===================================================================== */
@@ -129,6 +131,19 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 		abort();
 	}
 	const int ne = c->ne, nv = c->kp.nv;
+	/* GDN_VERBOSE: the wall time of each phase (the stream synchronized at
+	 * each mark) */
+	const bool verbose = getenv("GDN_VERBOSE") != nullptr;
+	auto t_mark = std::chrono::steady_clock::now();
+	auto mark = [&](const char *what) {
+		if (!verbose)
+			return;
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		const auto now = std::chrono::steady_clock::now();
+		fprintf(stderr, "gdon3d-hip: relaxation set-up, %s: %.3f s\n", what,
+			std::chrono::duration<double>(now - t_mark).count());
+		t_mark = now;
+	};
 	c->dt = dt;
 	c->nlevels = nlevels;
 	c->level_ptr.assign(level_ptr, level_ptr + nlevels + 1);
@@ -157,6 +172,7 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 	k_block_inverse<<<nblocks(c->nitems, 64), 64, 0, c->stream>>>(
 		c->items, c->nitems, c->geom, c->cls, c->kp, dt, c->dinv, c->bt);
 	HIPCHECK(hipGetLastError());
+	mark("items and block inverses");
 	/* lagged (cycle-breaking) inflow faces: capacitance form */
 	c->nlagged = 0;
 	for (size_t k = 0; k < (size_t)ne * 4 * nv; k++)
@@ -178,21 +194,26 @@ extern "C" void gdh_relax_setup(gdh_ctx *c, double dt, const uint8_t *cls,
 		HIPCHECK(hipStreamSynchronize(c->stream));
 		HIPCHECK(hipFree(dlast));
 	}
+	mark("item descriptors and residency");
 	const int mode = c->sweep_mode;
 	c->sweep_mode = mode == 0 ? 0 : 1; /* the capacitance set-up sweeps by items */
 	if (c->nlagged) {
 		const size_t blen = (size_t)c->m * c->neq;
 		c->wx = dmalloc(c->rsz * blen);
 		cap_setup(c);
+		mark("upwind cycles");
 	}
-	if (mode != 0 && mode != 1)
+	if (mode != 0 && mode != 1) {
 		cluster_setup(c, cls, items_ve, level_ptr, nlevels);
+		mark("cluster plan");
+	}
 	if (mode == 2 && !c->ncl) {
 		fprintf(stderr, "gdon3d-hip: GDN_SWEEP=cluster, but the mesh gives no cluster plan\n");
 		abort();
 	}
 	c->sweep_mode = mode;
 	relax_choose(c);
+	mark("form chosen by timing");
 }
 
 /* ------------------------------------------------------------------ */

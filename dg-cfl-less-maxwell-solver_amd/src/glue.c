@@ -4,6 +4,7 @@
  * callbacks, and context creation.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -41,6 +42,8 @@ void gdn_mesh_describe(const gdn_mesh *msh, gdh_mesh_desc *out)
 	double *xnode = xmalloc(sizeof(double) * 30 * (size_t)ne);
 	double *xghost = xmalloc(sizeof(double) * 18 * (size_t)(nb ? nb : 1));
 
+	/* elements on the host's threads: each writes only its own entries */
+#pragma omp parallel for schedule(static)
 	for (int e = 0; e < ne; e++) {
 		gdn_real codtau[3][3], det;
 		mesh_get_elem_codtau(msh, e, codtau);
@@ -493,13 +496,30 @@ gdh_ctx *gdn_ctx_open(gdn_simulation *simu, gdh_model_desc *md)
 
 gdh_ctx *gdn_ctx_open_ex(gdn_simulation *simu, gdh_model_desc *md, int precision)
 {
+	/* GDN_VERBOSE: the wall time of each phase */
+	const int verbose = getenv("GDN_VERBOSE") != NULL;
+	double t0 = omp_get_wtime();
+#define OPEN_MARK(what)                                                                  \
+	do {                                                                             \
+		if (verbose) {                                                           \
+			const double t1 = omp_get_wtime();                               \
+			fprintf(stderr, "gdon3d-hip: device context, %s: %.3f s\n", what, \
+				t1 - t0);                                                \
+			t0 = t1;                                                         \
+		}                                                                        \
+	} while (0)
 	if (gdh_device_count() < 1)
 		gdn_fatal("no HIP device is visible; the GPU solver cannot run");
+	OPEN_MARK("device query");
 	gdn_model_describe(simu->mdl, simu->use_kinetic_scheme, md);
 	gdh_mesh_desc d;
 	gdn_mesh_describe(simu->tt, &d);
+	OPEN_MARK("mesh arrays");
 	gdh_ctx *ctx = gdh_create_ex(&d, md, simu->use_kinetic_scheme ? 1 : 0, precision);
 	gdn_mesh_desc_release(&d);
+	OPEN_MARK("device buffers and uploads");
 	gdh_upload_state(ctx, simu->wn);
+	OPEN_MARK("state upload");
+#undef OPEN_MARK
 	return ctx;
 }

@@ -5,6 +5,7 @@
  * relaxation; final ghosts at t), split into create / init / advance /
  * finish so that drivers can time or inspect single steps.
  */
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -42,12 +43,26 @@ gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision)
 	if (!st)
 		gdn_fatal("out of memory");
 	st->simu = simu;
+	/* GDN_VERBOSE: the wall time of each phase */
+	const int verbose = getenv("GDN_VERBOSE") != NULL;
+	double t0 = omp_get_wtime();
+#define SETUP_MARK(what)                                                          \
+	do {                                                                      \
+		if (verbose) {                                                    \
+			const double t1 = omp_get_wtime();                        \
+			fprintf(stderr, "gdon3d-hip: set-up, %s: %.3f s\n", what, \
+				t1 - t0);                                         \
+			t0 = t1;                                                  \
+		}                                                                 \
+	} while (0)
 	st->ctx = gdn_ctx_open_ex(simu, &st->md,
 				  precision == GDN_PRECISION_FP32 ? GDH_FP32 : GDH_FP64);
+	SETUP_MARK("device context, mesh and geometry");
 	if (simu->use_kinetic_scheme) {
 		const gdn_mesh *msh = simu->tt;
 		const int ne = msh->nbelems, nv = simu->mdl->nb_v;
 		uint8_t *cls = gdn_upwind_classes(msh, simu->mdl);
+		SETUP_MARK("upwind classes");
 		gdh_mesh_desc d;
 		gdn_mesh_describe(msh, &d);
 		int *items = malloc(sizeof(int) * (size_t)nv * ne);
@@ -58,12 +73,15 @@ gdn_stepper *dg_stepper_create_precision(gdn_simulation *simu, int precision)
 		st->lagged = gdn_sweep_schedule(ne, nv, d.nbr, cls, items, lptr, &nl);
 		gdn_mesh_desc_release(&d);
 		st->levels = nl;
+		SETUP_MARK("sweep schedule");
 		gdh_relax_setup(st->ctx, simu->dt, cls, items, lptr, nl);
 		free(items);
 		free(lptr);
 		free(cls);
 	}
 	gdh_synchronize(st->ctx);
+	SETUP_MARK("relaxation set-up");
+#undef SETUP_MARK
 	return st;
 }
 
