@@ -586,6 +586,7 @@ def main():
             "sweep_form": sweep["form"],
             "sweep_clusters": sweep["clusters"],
             "sweep_resident_workgroups": sweep["grid"],
+            "setup_s": t_setup,
             "learned_inverses": (None if learned is None else
                                  {"items_on_network": learned[0], "items_on_stored": learned[1],
                                   "max_rel_deviation": learned[2]}),
