@@ -173,3 +173,41 @@ def test_cluster_plan_unstructured(mesh, kinetic):
     g = mesh_graph(os.path.join(os.path.dirname(__file__), "golden", mesh + ".msh"), kinetic)
     p = plan(g)
     check_plan(g, p)
+
+
+_PLAN_CHECKSUM = r"""
+import sys, hashlib
+for p in sys.argv[1:4]:
+    sys.path.insert(0, p)
+import test_cpu_cluster_plan as T
+g = T.mesh_graph(sys.argv[4], "d3q4")
+p = T.plan(g)
+h = hashlib.sha256()
+h.update(g["cls"].tobytes())
+for k in ("cls", "items", "lptr", "head", "perm", "code", "stats"):
+    h.update(p[k].tobytes())
+print(h.hexdigest())
+"""
+
+
+def test_setup_identical_for_any_thread_count():
+    """The upwind classes (per face), the sweep schedule with its lagged
+    cycle edges (per velocity) and the cluster plan (per velocity, face codes
+    over ranges of clusters) run on the host's threads: each is the same bit
+    for bit with 1, 3 and 8 threads, on a twisted mesh whose upwind graphs
+    have cycles."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    paths = [os.path.join(root, "dg-cfl-less-maxwell-solver_amd"), os.path.join(root, "oracle"),
+             here]
+    msh = twisted_msh(8, 2.5)
+    out = set()
+    for t in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        r = subprocess.run([sys.executable, "-c", _PLAN_CHECKSUM] + paths + [msh], env=env,
+                           check=True, capture_output=True, text=True)
+        out.add(r.stdout.strip().splitlines()[-1])
+    assert len(out) == 1
