@@ -375,6 +375,32 @@ int gdn_sweep_schedule(int nelem, int nv, const int *nbr, uint8_t *cls,
 	}
 	/* counting sort by level; (v, e) increasing inside a level */
 	const int nl = maxlev >= 0 ? maxlev + 1 : 0;
+	/* The items' order: each velocity's levels stretched over all nl
+	 * levels (L -> floor(L nl / nl_v): strictly increasing, so still a
+	 * topological order of each velocity's graph, and the velocities do
+	 * not depend on each other). A velocity with a shorter chain then
+	 * shares every level with the others instead of crowding the first
+	 * ones: on the Kuhn 32^3 cube one D3Q4 velocity has 94 levels, the
+	 * others 220 (persistent sweep with the relaxation fused 0.823 ->
+	 * 0.806 ms; the other forms unchanged; latest levels or the midpoint
+	 * of earliest and latest measured no better). GDN_SWEEP_ORDER=level
+	 * keeps the earliest levels. */
+	{
+		const char *o = getenv("GDN_SWEEP_ORDER");
+		if (!(o && !strcmp(o, "level")) && nl > 0) {
+			for (int v = 0; v < nv; v++) {
+				int *lev = level + (size_t)v * nelem;
+				int lv = 0;
+				for (int e = 0; e < nelem; e++)
+					if (lev[e] + 1 > lv)
+						lv = lev[e] + 1;
+				if (lv <= 0 || lv == nl)
+					continue;
+				for (int e = 0; e < nelem; e++)
+					lev[e] = (int)((long)lev[e] * nl / lv);
+			}
+		}
+	}
 	for (int L = 0; L <= nl; L++)
 		level_ptr[L] = 0;
 	for (int v = 0; v < nv; v++)
