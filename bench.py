@@ -49,12 +49,12 @@ SWEEP_BYTES_PER_ELEM = 480 * 8 + 112 + 16
 # (profiles/hbm_traffic_sweep_relax_rk3.txt): the sweep of one step at 32^3
 # D3Q4 and one RK3 stage at 64^3 jittered.
 SWEEP_TRAFFIC_BYTES_PER_ELEM = {  # (sweep form, relaxation fused into the sweep)
-    ("item", 1): (1728047.2 / 0.5 + 737742.8) * 1024 / 196608,
-    ("item", 0): (1034311.6 / 0.5 + 737837.0) * 1024 / 196608,
-    ("cluster", 0): (911277.1 / 0.5 + 646204.8) * 1024 / 196608,
+    ("item", 1): (1714364.3 / 0.5 + 737613.4) * 1024 / 196608,
+    ("item", 0): (1031197.7 / 0.5 + 737763.0) * 1024 / 196608,
+    ("cluster", 0): (915513.6 / 0.5 + 646294.6) * 1024 / 196608,
 }
 # the relaxation pass of one step (k_relax_pair<4, double>) at 32^3 D3Q4
-RELAX_TRAFFIC_BYTES_PER_ELEM = (184363.9 / 0.5 + 368911.1) * 1024 / 196608
+RELAX_TRAFFIC_BYTES_PER_ELEM = (184362.8 / 0.5 + 368892.1) * 1024 / 196608
 SWEEP_KERNEL = {"level": "k_sweep_level", "item": "k_sweep_persistent",
                 "cluster": "k_sweep_cluster"}
 # gdh_relax_fused: where the relaxation of a step is applied
